@@ -1,0 +1,204 @@
+/*
+ * TensorBLAS.h -- public interface of the MI355X (gfx950) Tensor-BLAS library.
+ *
+ * Drop-in for the reference's include/TensorBLAS.h:135-167.  Two layers:
+ *
+ *  1. The C ABI (extern "C", plain pointers and sizes).  Everything a foreign
+ *     binding (ctypes, cgo, JNI, N-API) needs:
+ *       - print_env, tc_syrk_wrapper, tc_trsm_wrapper: the reference's own C-ABI
+ *         exports, same signatures and meaning;
+ *       - tb_*: one entry point per routine with an explicit precision tier, which
+ *         replace the reference's C++-only tc_*, tc_cumpsgemm_*, tc_ozimmu_* families.
+ *  2. The reference's C++ entry points (tc_syrk, tc_cumpsgemm_syrk, ..., 14 of them,
+ *     plus tc_ozimmu_symm which the reference lacks) with the same argument order and
+ *     meaning.  The cuBLAS / cuMpSGEMM handle arguments become this library's handle
+ *     (tb_handle_t); the FP16 workspace arguments are accepted and may be NULL (the
+ *     library stages operands itself).
+ *
+ * Fixed options, as in the reference:
+ *   SYRK / SYR2K  C := alpha*A*A^T + beta*C (resp. alpha*(A*B^T + B*A^T) + beta*C),
+ *                 lower triangle written, A (and B) n x k, not transposed.
+ *   TRMM          C := alpha*L*B, left, lower, not transposed, non-unit, out of place.
+ *   TRSM          X*L^T = B, right, lower, transposed, non-unit, B overwritten by X.
+ *   SYMM          C := alpha*A*B + beta*C, left, A symmetric with its lower triangle stored.
+ * All matrices are column-major device pointers; dimensions are long.  Deliberate
+ * differences from the reference (documented at each function): the strictly upper
+ * triangle of a SYRK/SYR2K result is not written; TRMM ignores the upper triangle
+ * of L instead of requiring zeros; SYMM never modifies A; odd sizes run natively.
+ *
+ * Errors: every tb_* function returns TB_SUCCESS (0) or a TB_ERR_* code and
+ * launches nothing when it rejects its arguments.  The reference's tc_* returned
+ * void; here they return the same int status.  The library never prints.
+ *
+ * Streams: a handle owns one HIP stream (default: the null stream).  tb_* and tc_*
+ * calls are asynchronous on it; tc_syrk_wrapper / tc_trsm_wrapper synchronize before
+ * returning, as the reference's did (BLAS/tc_syrk.cu:29, tc_trsm.cu:18).
+ */
+#ifndef TENSORBLAS_MI355X_H
+#define TENSORBLAS_MI355X_H
+
+/* exported symbols; everything else in the library has hidden visibility */
+#define TB_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tb_handle_s* tb_handle_t;
+
+/* Precision tiers.  TB_TIER_PLAIN is the reference's tc_* family (FP32 operands
+ * converted to FP16 round-to-nearest-even, FP16 MFMA, FP32 accumulation);
+ * TB_TIER_SPLIT is the tc_cumpsgemm_* family (FP32-accurate: each FP32 operand is
+ * scaled per row by a power of two and split into FP16 hi + 2^-11 lo, three MFMA
+ * products); TB_TIER_OZAKI is the tc_ozimmu_* family (FP64 operands sliced into
+ * INT8 digits, INT8 MFMA with exact int32 sums, FP64 accumulation).  The two
+ * comparators have no reference family: TB_TIER_FP32 runs the same recursion on
+ * FP32 vector FMA, TB_TIER_FP64 on native FP64 MFMA. */
+enum {
+    TB_TIER_PLAIN = 0,
+    TB_TIER_SPLIT = 1,
+    TB_TIER_OZAKI = 2,
+    TB_TIER_FP32 = 3,
+    TB_TIER_FP64 = 4
+};
+
+enum {
+    TB_SUCCESS = 0,
+    TB_ERR_INVALID = 1,   /* bad dimension, leading dimension, nb, tier or null pointer */
+    TB_ERR_DEVICE = 2,    /* a HIP call failed (launch, allocation, copy) */
+    TB_ERR_NOMEM = 3      /* device workspace could not be allocated */
+};
+
+/* ------------------------------------------------------------- handle */
+TB_API int tb_create(tb_handle_t* handle);
+TB_API int tb_destroy(tb_handle_t handle);
+/* stream is a hipStream_t (NULL = null stream); streams keep the default priority */
+TB_API int tb_set_stream(tb_handle_t handle, void* stream);
+TB_API int tb_synchronize(tb_handle_t handle);
+/* Ozaki tier: number of INT8 slices (3..10, default 7) */
+TB_API int tb_set_ozaki_slices(tb_handle_t handle, int nslices);
+/* bytes of device workspace the handle currently holds */
+TB_API long tb_workspace_bytes(tb_handle_t handle);
+/* Per-launch profiling with HIP events on the handle's stream (off by default;
+ * switching it clears earlier records).  tb_profile_read synchronizes and writes,
+ * for each category c < ncat, out[3c] = launches, out[3c+1] = summed milliseconds,
+ * out[3c+2] = summed algorithmic flops.  Categories: 0..9 grouped GEMM
+ * (tier*2 + tile shape, shape 0 = 128x128 updates, 1 = 64x256 TRSM leaves),
+ * 10 Ozaki slice GEMM, 11 Ozaki slicing, 12 diagonal-block inversion. */
+#define TB_PROF_CATEGORIES 13
+TB_API int tb_set_profiling(tb_handle_t handle, int on);
+TB_API int tb_profile_read(tb_handle_t handle, double* out, int ncat);
+/* human-readable text of a status code */
+TB_API const char* tb_status_string(int status);
+/* library version: major*10000 + minor*100 + patch */
+TB_API int tb_version(void);
+
+/* ------------------------------------------------ routines (C ABI, tiered)
+ * Pointer types follow the tier: float* for PLAIN, SPLIT and FP32; double* for
+ * OZAKI and FP64.  alpha and beta are given in double and used at the tier's
+ * precision.  nb is the recursion / diagonal-block size of the reference (its
+ * README suggests 256 or 512); it changes the schedule, not the result, except
+ * for TRSM where it sets the leaf size. */
+
+/* replaces tc_syrk / tc_cumpsgemm_syrk / tc_ozimmu_syrk (include/TensorBLAS.h:139-143) */
+TB_API int tb_syrk(tb_handle_t h, int tier, long n, long k, double alpha, const void* A, long lda,
+            double beta, void* C, long ldc, long nb);
+/* replaces tc_syr2k / tc_cumpsgemm_syr2k / tc_ozimmu_syr2k (include/TensorBLAS.h:159-163) */
+TB_API int tb_syr2k(tb_handle_t h, int tier, long n, long k, double alpha, const void* A, long lda,
+             const void* B, long ldb, double beta, void* C, long ldc, long nb);
+/* replaces tc_trmm / tc_cumpsgemm_trmm / tc_ozimmu_trmm (include/TensorBLAS.h:153-157) */
+TB_API int tb_trmm(tb_handle_t h, int tier, long m, long n, double alpha, const void* A, long lda,
+            const void* B, long ldb, void* C, long ldc, long nb);
+/* replaces tc_trsm / tc_cumpsgemm_trsm / tc_ozimmu_trsm (include/TensorBLAS.h:147-151) */
+TB_API int tb_trsm(tb_handle_t h, int tier, long m, long n, const void* A, long lda, void* B, long ldb,
+            long nb);
+/* TRSM followed by one step of iterative refinement with an FP64 residual
+ * R = B - X*L^T computed on FP64 MFMA; the correction is solved in the same tier.
+ * FP32 tiers only.  No reference counterpart (added by the FP32-accuracy target). */
+TB_API int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* A, long lda, void* B,
+                    long ldb, long nb);
+/* replaces tc_symm / tc_cumpsgemm_symm (include/TensorBLAS.h:165-167); the OZAKI and
+ * FP64 tiers add the FP64 SYMM the reference does not have */
+TB_API int tb_symm(tb_handle_t h, int tier, long m, long n, double alpha, const void* A, long lda,
+            const void* B, long ldb, double beta, void* C, long ldc);
+/* General C := alpha*op(A)*op(B) + beta*C (transa/transb: 0 = N, 1 = T); the update
+ * GEMM every routine above is built from (the reference's cublasGemmEx call sites,
+ * BLAS/tc_syrk.cu:106-118). */
+TB_API int tb_gemm(tb_handle_t h, int tier, int transa, int transb, long m, long n, long k, double alpha,
+            const void* A, long lda, const void* B, long ldb, double beta, void* C, long ldc);
+
+/* ---------------------------------- conversion kernels (util/util.cu:127-167) */
+/* ah = fp16(as), m x n (RNE); the reference's s2h / s2h_swpipe */
+TB_API int tb_s2h(tb_handle_t h, long m, long n, const float* as, long ldas, void* ah, long ldah);
+/* row-major m x n FP32 -> column-major FP16; the reference's s2hTranspose */
+TB_API int tb_s2h_transpose(tb_handle_t h, long m, long n, const float* as, void* ah);
+/* A(i,j) := A(j,i), i < j; the reference's copy_lower_to_upper{,Double} */
+TB_API int tb_copy_lower_to_upper(tb_handle_t h, int f64, long n, void* A, long lda);
+
+/* ------------------------------------------------ schedule inspection (host only)
+ * Chunks of n in processing order as util/mat_size.cu:5-55 computes them
+ * (find_mat_size_syrk when trsm_planner == 0, find_mat_size_trsm otherwise),
+ * continued with 16384s past the reference's table.  Returns the count, or
+ * -TB_ERR_INVALID when cap is too small.  Launches nothing. */
+TB_API int tb_plan_chunks(int trsm_planner, long n, long* out, int cap);
+/* The TRSM schedule of BLAS/tc_trsm.cu:31-137 for an n-column problem with block nb:
+ * 5 longs per step, in execution order: (0, off, s, 0, 0) a diagonal leaf of size s,
+ * (1, c0, w, k0, kw) an update B(:,c0:c0+w) -= B(:,k0:k0+kw)*L(c0:c0+w,k0:k0+kw)^T.
+ * Returns the step count or -TB_ERR_INVALID.  Launches nothing. */
+TB_API int tb_plan_trsm(long n, long nb, long* out, int cap);
+
+/* ---------------------------------------- the reference's own C-ABI exports */
+/* include/TensorBLAS.h:135 -- prints the device properties */
+TB_API void print_env(void);
+/* include/TensorBLAS.h:137, BLAS/tc_syrk.cu:4-39: A is n x k ROW-major (a torch
+ * tensor), C (n x n) receives the full symmetric A*A^T (plain tier).  Returns 0 on
+ * success, 1 on error; synchronous. */
+TB_API int tc_syrk_wrapper(long n, long k, float* A, float* C, long nb);
+/* include/TensorBLAS.h:145, BLAS/tc_trsm.cu:4-28: solves X*A^T = B in place for
+ * column-major A (n x n, lda = n) and B (m x n, ldb = m), plain tier.  Returns 0/1;
+ * synchronous. */
+TB_API int tc_trsm_wrapper(long m, long n, float* A, float* B, long nb);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+/* ----------------------------------------- the reference's C++ entry points
+ * include/TensorBLAS.h:139-167 with tb_handle_t in place of cublasHandle_t and
+ * cumpsgemm::handle_t.  __half* workspaces are accepted (NULL allowed) and unused. */
+struct __half;
+
+TB_API int tc_syrk(tb_handle_t handle, long n, long k, float alpha, float* A, long lda, float beta,
+            float* C, long ldc, __half* Ah, long nb);
+TB_API int tc_cumpsgemm_syrk(tb_handle_t handle, long n, long k, float alpha, float* A, long lda,
+                      float beta, float* C, long ldc, long nb);
+TB_API int tc_ozimmu_syrk(tb_handle_t handle, long n, long k, double alpha, double* A, long lda,
+                   double beta, double* C, long ldc, long nb);
+TB_API int tc_trsm(tb_handle_t handle, long m, long n, float* A, long lda, float* B, long ldb,
+            __half* hwork, long nb);
+/* the reference takes both a cuBLAS and a cuMpSGEMM handle here */
+TB_API int tc_cumpsgemm_trsm(tb_handle_t handle, tb_handle_t handle2, long m, long n, float* A, long lda,
+                      float* B, long ldb, long nb);
+TB_API int tc_ozimmu_trsm(tb_handle_t handle, long m, long n, double* A, long lda, double* B, long ldb,
+                   long nb);
+TB_API int tc_trmm(tb_handle_t handle, long m, long n, float alpha, float* A, long lda, float* B,
+            long ldb, float* C, long ldc, __half* hwork, long nb);
+TB_API int tc_cumpsgemm_trmm(tb_handle_t handle, long m, long n, float alpha, float* A, long lda,
+                      float* B, long ldb, float* C, long ldc, long nb);
+TB_API int tc_ozimmu_trmm(tb_handle_t handle, long m, long n, double alpha, double* A, long lda,
+                   double* B, long ldb, double* C, long ldc, long nb);
+TB_API int tc_syr2k(tb_handle_t handle, long n, long k, float alpha, float* A, long lda, float* B,
+             long ldb, float beta, float* C, long ldc, __half* Ah, long nb);
+TB_API int tc_cumpsgemm_syr2k(tb_handle_t handle, long n, long k, float alpha, float* A, long lda,
+                       float* B, long ldb, float beta, float* C, long ldc, long nb);
+TB_API int tc_ozimmu_syr2k(tb_handle_t handle, long n, long k, double alpha, double* A, long lda,
+                    double* B, long ldb, double beta, double* C, long ldc, long nb);
+TB_API int tc_symm(tb_handle_t handle, long m, long n, float alpha, float* A, long lda, float* B,
+            int ldb, float beta, float* C, long ldc, __half* Ah);
+TB_API int tc_cumpsgemm_symm(tb_handle_t handle, long m, long n, float alpha, float* A, long lda,
+                      float* B, int ldb, float beta, float* C, long ldc);
+/* no reference counterpart: the FP64 SYMM named by the Ozaki benchmark */
+TB_API int tc_ozimmu_symm(tb_handle_t handle, long m, long n, double alpha, double* A, long lda,
+                   double* B, long ldb, double beta, double* C, long ldc);
+#endif
+
+#endif /* TENSORBLAS_MI355X_H */

@@ -1,0 +1,444 @@
+// Auxiliary device kernels: row exponents for the split and Ozaki tiers, inverses
+// of the TRSM diagonal blocks, and the FP32->FP16 / mirror / fill / copy kernels of
+// the reference's util/util.cu:127-280 and include/TensorBLAS.h:185-284.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tb_kernels.hpp"
+
+namespace tb {
+
+namespace {
+
+__device__ __forceinline__ int expo(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    int b = (u >> 23) & 0xff;
+    if (b) return b - 126;
+    if ((u & 0x7fffffffu) == 0) return kExpZero;
+    int e;
+    frexpf(x, &e);
+    return e;
+}
+
+__device__ __forceinline__ int expo(double x)
+{
+    uint64_t u = (uint64_t)__double_as_longlong(x);
+    int b = (int)((u >> 52) & 0x7ff);
+    if (b) return b - 1022;
+    if ((u & 0x7fffffffffffffffull) == 0) return kExpZero;
+    int e;
+    frexp(x, &e);
+    return e;
+}
+
+constexpr int kRowChunk = 256;    // k per block of the row-major pass
+constexpr int kColChunk = 1024;   // k per block of the k-contiguous pass
+
+// rows contiguous across threads: one thread per row, k split over blockIdx.y
+template <typename T>
+__global__ void __launch_bounds__(256)
+rowexp_mn_kernel(const T* __restrict__ X, long long ld, int rows, int K, bool lower_only,
+                 long long sx, int* out, long long so)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    const T* Xb = X + (long long)blockIdx.z * sx;
+    const int k0 = blockIdx.y * kRowChunk;
+    int k1 = min(K, k0 + kRowChunk);
+    if (lower_only) k1 = min(k1, r + 1);
+    int e = kExpZero;
+    for (int k = k0; k < k1; ++k) e = max(e, expo(Xb[(long long)r + (long long)k * ld]));
+    if (e != kExpZero) atomicMax(out + (long long)blockIdx.z * so + r, e);
+}
+
+// k contiguous: one wave per row, lanes across k; kfrom_diag starts row r at k = r+1
+template <typename T>
+__global__ void __launch_bounds__(256)
+rowexp_k_kernel(const T* __restrict__ X, long long ld, int rows, int K, bool kfrom_diag,
+                long long sx, int* out, long long so)
+{
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= rows) return;
+    const T* Xb = X + (long long)blockIdx.z * sx + (long long)r * ld;
+    int k0 = blockIdx.y * kColChunk;
+    const int k1 = min(K, k0 + kColChunk);
+    if (kfrom_diag) k0 = max(k0, r + 1);
+    int e = kExpZero;
+    for (int k = k0 + lane; k < k1; k += 64) e = max(e, expo(Xb[k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
+    if (lane == 0 && e != kExpZero) atomicMax(out + (long long)blockIdx.z * so + r, e);
+}
+
+__global__ void fill_i32_kernel(int* p, long long n, int v)
+{
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) p[i] = v;
+}
+
+// Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.
+// Writes W = D inv(L) - I = (I+N)^-1 - I (strictly lower) as Z = W^T (Z[c + r*s] =
+// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One wave owns 64 columns of
+// inv(L): lane c computes column cb+c by forward substitution, keeping the column in
+// LDS.  Row r of L is staged in LDS by the whole wave first, so the inner product
+// reads it as a broadcast.
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(64)
+trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride, int s,
+                     Tout* __restrict__ Z, double* __restrict__ dinv)
+{
+    __shared__ double y[256 * 64];
+    __shared__ double lrow[256];
+    const Tin* Lq = L + (long long)blockIdx.y * bstride;
+    Tout* Zq = Z + (long long)blockIdx.y * s * s;
+    const int cb = blockIdx.x * 64;
+    const int lane = threadIdx.x;
+    const int col = cb + lane;
+    const bool live = col < s;
+    if (blockIdx.x == 0)
+        for (int r = lane; r < s; r += 64) dinv[(long long)blockIdx.y * s + r] = 1.0 / (double)Lq[r + (long long)r * lda];
+    for (int r = 0; r < cb; ++r)
+        if (live) Zq[col + (long long)r * s] = Tout(0);
+    for (int r = cb; r < s; ++r) {
+        for (int p = cb + lane; p <= r; p += 64) lrow[p - cb] = (double)Lq[r + (long long)p * lda];
+        __syncthreads();
+        double sum = (r == col) ? 1.0 : 0.0;
+        for (int p = cb; p < r; ++p) sum -= lrow[p - cb] * y[(p - cb) * 64 + lane];
+        const double d = lrow[r - cb];
+        const double v = (r < col || !live) ? 0.0 : sum / d;
+        y[(r - cb) * 64 + lane] = v;
+        // W(r, col) = L(r,r) * inv(L)(r, col) for r > col
+        if (live) Zq[col + (long long)r * s] = (r > col) ? (Tout)(d * v) : Tout(0);
+        __syncthreads();
+    }
+}
+
+// FP32 -> FP16, round to nearest even
+__global__ void __launch_bounds__(256)
+s2h_kernel(long long m, long long n, const float* __restrict__ as, long long ldas,
+           _Float16* __restrict__ ah, long long ldah)
+{
+    for (long long j = blockIdx.y; j < n; j += gridDim.y)
+        for (long long i = blockIdx.x * 256ll + threadIdx.x; i < m; i += 256ll * gridDim.x)
+            ah[i + j * ldah] = (_Float16)as[i + j * ldas];
+}
+
+// row-major m x n FP32 -> column-major m x n FP16 through a 32x32 LDS tile
+__global__ void __launch_bounds__(256)
+s2h_transpose_kernel(long long m, long long n, const float* __restrict__ as,
+                     _Float16* __restrict__ ah)
+{
+    __shared__ float t[32][33];
+    const long long i0 = blockIdx.y * 32ll, j0 = blockIdx.x * 32ll;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int q = ty; q < 32; q += 8) {
+        long long i = i0 + q, j = j0 + tx;                 // read row i, contiguous j
+        if (i < m && j < n) t[q][tx] = as[i * n + j];
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        long long j = j0 + q, i = i0 + tx;                 // write column j, contiguous i
+        if (i < m && j < n) ah[i + j * m] = (_Float16)t[tx][q];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+mirror_lower_kernel(long long n, T* A, long long lda)
+{
+    __shared__ T t[32][33];
+    const long long bi = blockIdx.y, bj = blockIdx.x;      // destination tile (bi, bj), bi <= bj
+    if (bi > bj) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // source tile (bj, bi) in the lower triangle
+    for (int q = ty; q < 32; q += 8) {
+        long long i = bj * 32 + tx, j = bi * 32 + q;
+        if (i < n && j < n) t[q][tx] = A[i + j * lda];
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        long long i = bi * 32 + tx, j = bj * 32 + q;      // A(i, j) := A(j, i) for i < j
+        if (i < n && j < n && i < j) A[i + j * lda] = t[tx][q];
+    }
+}
+
+template <typename T>
+__global__ void fill_matrix_kernel(long long m, long long n, T* a, long long lda, T v)
+{
+    for (long long j = blockIdx.y; j < n; j += gridDim.y)
+        for (long long i = blockIdx.x * 256ll + threadIdx.x; i < m; i += 256ll * gridDim.x)
+            a[i + j * lda] = v;
+}
+
+template <typename TA, typename TB>
+__global__ void copy_matrix_kernel(long long m, long long n, const TA* a, long long lda, TB* b,
+                                   long long ldb)
+{
+    for (long long j = blockIdx.y; j < n; j += gridDim.y)
+        for (long long i = blockIdx.x * 256ll + threadIdx.x; i < m; i += 256ll * gridDim.x)
+            b[i + j * ldb] = (TB)a[i + j * lda];
+}
+
+template <typename TA, typename TB>
+__global__ void copy_lower_kernel(long long n, const TA* a, long long lda, TB* b, long long ldb)
+{
+    for (long long j = blockIdx.y; j < n; j += gridDim.y)
+        for (long long i = j + blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x)
+            b[i + j * ldb] = (TB)a[i + j * lda];
+}
+
+__global__ void add_matrix_f32_kernel(long long m, long long n, float* x, long long ldx,
+                                      const float* y, long long ldy)
+{
+    for (long long j = blockIdx.y; j < n; j += gridDim.y)
+        for (long long i = blockIdx.x * 256ll + threadIdx.x; i < m; i += 256ll * gridDim.x)
+            x[i + j * ldx] += y[i + j * ldy];
+}
+
+// 64 rows x 64 plane columns per block through an LDS tile, so both the source read
+// (rows or k contiguous) and the 16-byte plane writes are coalesced.
+template <int NPL>
+__global__ void __launch_bounds__(256)
+split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int rows, int K,
+                    const int* __restrict__ e, _Float16* __restrict__ out, long long ldp,
+                    long long sp, int kshift)
+{
+    __shared__ float t[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;   // c: plane column = kshift + k
+    const int tid = threadIdx.x;
+    for (int q = tid; q < 64 * 64; q += 256) {
+        int rl, cl;
+        if (mode == kOpT) { cl = q % 64; rl = q / 64; }
+        else { rl = q % 64; cl = q / 64; }
+        const int r = r0 + rl, k = c0 + cl - kshift;
+        float v = 0.0f;
+        if (r < rows && k >= 0 && k < K) {
+            if (mode == kOpN) v = X[(long long)r + (long long)k * ld];
+            else if (mode == kOpT) v = X[(long long)k + (long long)r * ld];
+            else if (r >= k) v = X[(long long)r + (long long)k * ld];
+            else if (mode == kOpSymL) v = X[(long long)k + (long long)r * ld];
+        }
+        t[rl][cl] = v;
+    }
+    __syncthreads();
+    const int rl = tid >> 2, cl0 = (tid & 3) * 16;
+    const int r = r0 + rl;
+    if (r >= rows) return;
+    int sc = 0;
+    if (NPL == 3) {
+        const int ex = e[r];
+        sc = 15 - (ex < -1000 ? 0 : (ex > 1000 ? 1000 : ex));
+    }
+    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    for (int half = 0; half < 2; ++half) {
+        half8 hh, mm, ll;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float x = t[rl][cl0 + half * 8 + j];
+            if (NPL == 1) {
+                hh[j] = (_Float16)x;
+            } else {
+                const float xs = __builtin_ldexpf(x, sc);
+                const _Float16 h = (_Float16)xs;
+                const float r1 = (xs - (float)h) * 2048.0f;
+                const _Float16 m = (_Float16)r1;
+                const float r2 = (r1 - (float)m) * 2048.0f;
+                hh[j] = h;
+                mm[j] = m;
+                ll[j] = (_Float16)r2;
+            }
+        }
+        _Float16* dst = out + (long long)r * ldp + c0 + cl0 + half * 8;
+        *reinterpret_cast<half8*>(dst) = hh;
+        if (NPL == 3) {
+            *reinterpret_cast<half8*>(dst + sp) = mm;
+            *reinterpret_cast<half8*>(dst + 2 * sp) = ll;
+        }
+    }
+}
+
+dim3 grid2d(long long m, long long n)
+{
+    long long gx = (m + 255) / 256;
+    if (gx > 65535) gx = 65535;
+    long long gy = n < 65535 ? n : 65535;
+    if (gx < 1) gx = 1;
+    if (gy < 1) gy = 1;
+    return dim3((unsigned)gx, (unsigned)gy);
+}
+
+}  // namespace
+
+hipError_t split_planes(int npl, const float* X, long long ld, int mode, int rows, int K,
+                        const int* e, _Float16* out, long long ldp, long long sp, int kshift,
+                        hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    if (ldp % 64 || kshift < 0 || ldp < kshift + K || (npl != 1 && npl != 3) ||
+        (reinterpret_cast<uintptr_t>(out) % 16) || (npl == 3 && sp % 8))
+        return hipErrorInvalidValue;
+    dim3 g((unsigned)(ldp / 64), (unsigned)((rows + 63) / 64));
+    if (npl == 1)
+        hipLaunchKernelGGL(split_planes_kernel<1>, g, dim3(256), 0, s, X, ld, mode, rows, K, e, out,
+                           ldp, sp, kshift);
+    else
+        hipLaunchKernelGGL(split_planes_kernel<3>, g, dim3(256), 0, s, X, ld, mode, rows, K, e, out,
+                           ldp, sp, kshift);
+    return hipGetLastError();
+}
+
+hipError_t fill_i32(int* p, long long n, int v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long long g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t rowexp(bool f64, const void* X, long long ld, int mode, int rows, int K, int nb,
+                  long long sx, int* out, long long so, hipStream_t s)
+{
+    if (rows <= 0 || K <= 0 || nb <= 0) return hipSuccess;
+    if (mode == kOpN || mode == kOpTriL || mode == kOpSymL) {
+        dim3 g((rows + 255) / 256, (K + kRowChunk - 1) / kRowChunk, nb);
+        const bool lower = mode != kOpN;
+        if (f64)
+            hipLaunchKernelGGL(rowexp_mn_kernel<double>, g, dim3(256), 0, s,
+                               (const double*)X, ld, rows, K, lower, sx, out, so);
+        else
+            hipLaunchKernelGGL(rowexp_mn_kernel<float>, g, dim3(256), 0, s,
+                               (const float*)X, ld, rows, K, lower, sx, out, so);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || mode != kOpSymL) return e;
+    }
+    if (mode == kOpT || mode == kOpSymL) {
+        // kOpSymL: the entries right of the diagonal of row r are column r below it
+        dim3 g((rows + 3) / 4, (K + kColChunk - 1) / kColChunk, nb);
+        const bool from_diag = mode == kOpSymL;
+        if (f64)
+            hipLaunchKernelGGL(rowexp_k_kernel<double>, g, dim3(256), 0, s,
+                               (const double*)X, ld, rows, K, from_diag, sx, out, so);
+        else
+            hipLaunchKernelGGL(rowexp_k_kernel<float>, g, dim3(256), 0, s,
+                               (const float*)X, ld, rows, K, from_diag, sx, out, so);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
+hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda,
+                         long long bstride, int s, int nblocks, void* Z, double* dinv,
+                         hipStream_t st)
+{
+    if (s <= 0 || nblocks <= 0) return hipSuccess;
+    if (s > 256) return hipErrorInvalidValue;
+    dim3 g((s + 63) / 64, nblocks);
+    if (in_f64 && out_f64)
+        hipLaunchKernelGGL((trtri_lower_t_kernel<double, double>), g, dim3(64), 0, st,
+                           (const double*)L, lda, bstride, s, (double*)Z, dinv);
+    else if (in_f64)
+        hipLaunchKernelGGL((trtri_lower_t_kernel<double, float>), g, dim3(64), 0, st,
+                           (const double*)L, lda, bstride, s, (float*)Z, dinv);
+    else if (out_f64)
+        hipLaunchKernelGGL((trtri_lower_t_kernel<float, double>), g, dim3(64), 0, st,
+                           (const float*)L, lda, bstride, s, (double*)Z, dinv);
+    else
+        hipLaunchKernelGGL((trtri_lower_t_kernel<float, float>), g, dim3(64), 0, st,
+                           (const float*)L, lda, bstride, s, (float*)Z, dinv);
+    return hipGetLastError();
+}
+
+hipError_t s2h(long long m, long long n, const float* as, long long ldas, void* ah, long long ldah,
+               hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(s2h_kernel, grid2d(m, n), dim3(256), 0, s, m, n, as, ldas, (_Float16*)ah,
+                       ldah);
+    return hipGetLastError();
+}
+
+hipError_t s2h_transpose(long long m, long long n, const float* as, void* ah, hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    dim3 g((unsigned)((n + 31) / 32), (unsigned)((m + 31) / 32));
+    hipLaunchKernelGGL(s2h_transpose_kernel, g, dim3(256), 0, s, m, n, as, (_Float16*)ah);
+    return hipGetLastError();
+}
+
+hipError_t mirror_lower(bool f64, long long n, void* A, long long lda, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    dim3 g((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+    if (f64)
+        hipLaunchKernelGGL(mirror_lower_kernel<double>, g, dim3(256), 0, s, n, (double*)A, lda);
+    else
+        hipLaunchKernelGGL(mirror_lower_kernel<float>, g, dim3(256), 0, s, n, (float*)A, lda);
+    return hipGetLastError();
+}
+
+hipError_t fill_matrix(bool f64, long long m, long long n, void* a, long long lda, double v,
+                       hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (f64)
+        hipLaunchKernelGGL(fill_matrix_kernel<double>, grid2d(m, n), dim3(256), 0, s, m, n,
+                           (double*)a, lda, v);
+    else
+        hipLaunchKernelGGL(fill_matrix_kernel<float>, grid2d(m, n), dim3(256), 0, s, m, n,
+                           (float*)a, lda, (float)v);
+    return hipGetLastError();
+}
+
+hipError_t copy_matrix(bool a_f64, bool b_f64, long long m, long long n, const void* a,
+                       long long lda, void* b, long long ldb, hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    dim3 g = grid2d(m, n);
+    if (a_f64 && b_f64)
+        hipLaunchKernelGGL((copy_matrix_kernel<double, double>), g, dim3(256), 0, s, m, n,
+                           (const double*)a, lda, (double*)b, ldb);
+    else if (a_f64)
+        hipLaunchKernelGGL((copy_matrix_kernel<double, float>), g, dim3(256), 0, s, m, n,
+                           (const double*)a, lda, (float*)b, ldb);
+    else if (b_f64)
+        hipLaunchKernelGGL((copy_matrix_kernel<float, double>), g, dim3(256), 0, s, m, n,
+                           (const float*)a, lda, (double*)b, ldb);
+    else
+        hipLaunchKernelGGL((copy_matrix_kernel<float, float>), g, dim3(256), 0, s, m, n,
+                           (const float*)a, lda, (float*)b, ldb);
+    return hipGetLastError();
+}
+
+hipError_t copy_lower(bool a_f64, bool b_f64, long long n, const void* a, long long lda, void* b,
+                      long long ldb, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    dim3 g = grid2d(n, n);
+    if (a_f64 && b_f64)
+        hipLaunchKernelGGL((copy_lower_kernel<double, double>), g, dim3(256), 0, s, n,
+                           (const double*)a, lda, (double*)b, ldb);
+    else if (a_f64)
+        hipLaunchKernelGGL((copy_lower_kernel<double, float>), g, dim3(256), 0, s, n,
+                           (const double*)a, lda, (float*)b, ldb);
+    else if (b_f64)
+        hipLaunchKernelGGL((copy_lower_kernel<float, double>), g, dim3(256), 0, s, n,
+                           (const float*)a, lda, (double*)b, ldb);
+    else
+        hipLaunchKernelGGL((copy_lower_kernel<float, float>), g, dim3(256), 0, s, n,
+                           (const float*)a, lda, (float*)b, ldb);
+    return hipGetLastError();
+}
+
+hipError_t add_matrix_f32(long long m, long long n, void* x, long long ldx, const float* y,
+                          long long ldy, hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(add_matrix_f32_kernel, grid2d(m, n), dim3(256), 0, s, m, n, (float*)x, ldx,
+                       y, ldy);
+    return hipGetLastError();
+}
+
+}  // namespace tb

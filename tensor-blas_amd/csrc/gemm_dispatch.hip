@@ -1,0 +1,72 @@
+// Instantiations of the grouped GEMM kernels and the host-side dispatcher.
+#include "gemm16_kernel.hpp"
+#include "gemm_kernel.hpp"
+#include "tb_kernels.hpp"
+
+namespace tb {
+
+namespace {
+
+template <int TIER, int BM, int BN, int BK, int WM, int WN>
+hipError_t run(const GemmLaunch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL((gemm_grouped_kernel<TIER, BM, BN, BK, WM, WN>), dim3(L.totalTiles),
+                       dim3(kThreads), 0, s, L);
+    return hipGetLastError();
+}
+
+template <int NPL, int SA, int SB, int BM, int BN, int BK, int WM, int WN>
+hipError_t run16(const GemmLaunch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL((gemm16_kernel<NPL, SA, SB, BM, BN, BK, WM, WN>), dim3(L.totalTiles),
+                       dim3(64 * WM * WN), 0, s, L);
+    return hipGetLastError();
+}
+
+template <int NPL, int SA, int SB>
+hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s)
+{
+    return shape == kShapeLeaf ? run16<NPL, SA, SB, 64, 256, 32, 2, 4>(L, s)
+                               : run16<NPL, SA, SB, 128, 128, 32, 4, 2>(L, s);
+}
+
+template <int NPL>
+hipError_t run16_src(int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s)
+{
+    if (srcA == kSrcF32 && srcB == kSrcF32) return run16_shape<NPL, kSrcF32, kSrcF32>(shape, L, s);
+    if (srcA == kSrcF32 && srcB == kSrcPlanes) return run16_shape<NPL, kSrcF32, kSrcPlanes>(shape, L, s);
+    if (srcA == kSrcPlanes && srcB == kSrcPlanes)
+        return run16_shape<NPL, kSrcPlanes, kSrcPlanes>(shape, L, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+TileShape gemm_tile_shape(int tier, int shape)
+{
+    if (shape == kShapeLeaf) return TileShape{64, 256};
+    return TileShape{128, 128};
+}
+
+int tier_planes(int tier) { return tier == kSplitF16 ? 3 : (tier == kPlainF16 ? 1 : 0); }
+
+hipError_t gemm_launch(int tier, int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s)
+{
+    if (L.totalTiles <= 0) return hipSuccess;
+    switch (tier) {
+    case kPlainF16: return run16_src<1>(shape, srcA, srcB, L, s);
+    case kSplitF16: return run16_src<3>(shape, srcA, srcB, L, s);
+    case kFmaF32:
+        if (srcA != kSrcF32 || srcB != kSrcF32) return hipErrorInvalidValue;
+        return shape == kShapeLeaf ? run<kFmaF32, 64, 256, 32, 1, 4>(L, s)
+                                   : run<kFmaF32, 128, 128, 32, 2, 2>(L, s);
+    case kMfmaF64:
+        if (srcA != kSrcF32 || srcB != kSrcF32) return hipErrorInvalidValue;
+        return shape == kShapeLeaf ? run<kMfmaF64, 64, 256, 16, 1, 4>(L, s)
+                                   : run<kMfmaF64, 128, 128, 16, 2, 2>(L, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace tb

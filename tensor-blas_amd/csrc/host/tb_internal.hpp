@@ -1,0 +1,120 @@
+// Host-side internals: the handle, its device workspace, the GEMM problem queue and
+// the routine planners (tb_blas3.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../../include/TensorBLAS.h"
+#include "../tb_kernels.hpp"
+
+// One profiled launch: events recorded on the handle's stream around it.
+struct tb_prof_rec {
+    int cat;          // tb::prof_category
+    hipEvent_t e0, e1;
+    double flops;     // algorithmic flops of the launch
+};
+
+struct tb_handle_s {
+    hipStream_t stream = nullptr;
+    bool prof = false;
+    std::vector<tb_prof_rec> prof_recs;
+    int device = 0;
+    int ozaki_slices = 7;
+    // device workspaces, grown on demand, freed by tb_destroy: slot 0 for routines,
+    // slot 1 for the Ozaki GEMM (which routines call with their own slot live)
+    void* ws[2] = {nullptr, nullptr};
+    size_t ws_bytes[2] = {0, 0};
+};
+
+namespace tb {
+
+// Profiling categories: grouped GEMM (tier*2 + shape), then the other kernels.
+enum ProfCat : int {
+    kProfGemm0 = 0,            // .. kProfGemm0 + 9: tier * 2 + shape
+    kProfOzakiGemm = 10,
+    kProfOzakiSlice = 11,
+    kProfTrtri = 12,
+    kProfCats = 13,
+};
+// Wraps one launch in events when the handle profiles.
+int prof_begin(tb_handle_s* h, hipEvent_t* e0);
+int prof_end(tb_handle_s* h, int cat, hipEvent_t e0, double flops);
+
+// status helpers
+inline int hip_status(hipError_t e) { return e == hipSuccess ? TB_SUCCESS : TB_ERR_DEVICE; }
+#define TB_TRY(expr)                          \
+    do {                                      \
+        int _st = (expr);                     \
+        if (_st != TB_SUCCESS) return _st;    \
+    } while (0)
+#define TB_HIP(expr) TB_TRY(::tb::hip_status(expr))
+
+// Workspace carving: sizes are reserved first, then the handle's buffer is grown
+// once (synchronizing its stream before freeing a smaller one).
+class Arena {
+public:
+    size_t reserve(size_t bytes)
+    {
+        size_t off = (total_ + 255) & ~size_t(255);
+        total_ = off + bytes;
+        return off;
+    }
+    size_t total() const { return total_; }
+
+private:
+    size_t total_ = 0;
+};
+int workspace_get(tb_handle_s* h, size_t bytes, char** base, int slot = 0);
+
+// Collects GEMM problems of one tier, tile shape and operand source kinds (all
+// problems must match them); launches up to kMaxProbs problems per kernel launch,
+// in order.
+class GemmQueue {
+public:
+    GemmQueue(tb_handle_s* h, int tier, int shape, int srcA = 0, int srcB = 0);
+    int add(const GemmProb& p);
+    int flush();
+
+private:
+    tb_handle_s* h_;
+    int tier_, shape_, srcA_, srcB_;
+    TileShape ts_;
+    GemmLaunch L_;
+};
+
+GemmProb make_prob(long m, long n, long k, const void* A, long lda, int modeA, const void* B,
+                   long ldb, int modeB, void* C, long ldc, double alpha, double beta, int modeC);
+
+// Chunk planners: the reference's util/mat_size.cu:5-55, continued with 16384s past
+// the end of its power table (where the reference reads out of bounds).
+std::vector<long> plan_chunks_syrk(long n);
+std::vector<long> plan_chunks_trsm(long n);
+
+// routines (argument validation included); trans_a: op(A) = A^T with A stored k x n
+int syrk(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A, long lda,
+         bool trans_a, double beta, void* C, long ldc, long nb);
+int syr2k(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A, long lda,
+          const void* B, long ldb, double beta, void* C, long ldc, long nb);
+int trmm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, long lda,
+         const void* B, long ldb, void* C, long ldc, long nb);
+int trsm(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B, long ldb,
+         long nb);
+int trsm_refined(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B,
+                 long ldb, long nb);
+// schedule of TRSM (leaves and updates in execution order), for inspection
+int plan_trsm(long n, long nb, long* out, int cap);
+int symm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, long lda,
+         const void* B, long ldb, double beta, void* C, long ldc);
+int gemm(tb_handle_s* h, int tier, int transa, int transb, long m, long n, long k, double alpha,
+         const void* A, long lda, const void* B, long ldb, double beta, void* C, long ldc);
+
+// Ozaki tier (tb_ozaki.cpp): C = alpha*op(A)*op(B) (+ op(A2)*op(B2)) + beta*C in FP64.
+// The second pair is SYR2K's swap: A2 is read with ldb and B2 with lda.
+int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const double* A,
+                    long lda, int modeA, const double* B, long ldb, int modeB, const double* A2,
+                    const double* B2, double beta, double* C, long ldc, bool lowerC);
+
+}  // namespace tb

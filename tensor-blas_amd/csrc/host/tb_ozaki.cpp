@@ -1,0 +1,89 @@
+// Host side of the Ozaki INT8 tier: exponents, slicing, the slice GEMM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "tb_internal.hpp"
+
+namespace tb {
+
+namespace {
+int flip_mode(int mode) { return mode == kOpT ? kOpN : (mode == kOpN ? kOpT : mode); }
+}  // namespace
+
+// K is processed in chunks so that every int32 sum D_g stays exact:
+// nslices * kchunk * 127^2 < 2^31 (all pairs of one g share an accumulator).
+long ozaki_kchunk(int nslices)
+{
+    long kc = ((1l << 31) - 1) / ((long)nslices * 127 * 127);
+    return kc / 64 * 64;
+}
+
+int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const double* A,
+                    long lda, int modeA, const double* B, long ldb, int modeB, const double* A2,
+                    const double* B2, double beta, double* C, long ldc, bool lowerC)
+{
+    if (m <= 0 || n <= 0) return TB_SUCCESS;
+    const int S = h->ozaki_slices;
+    const long kpad = std::max(64l, (k + 63) / 64 * 64);
+    const bool pairs = A2 != nullptr;
+    Arena a;
+    const size_t oEA = a.reserve(sizeof(int) * m);
+    const size_t oEB = a.reserve(sizeof(int) * n);
+    const size_t oSA = a.reserve((size_t)S * m * kpad);
+    const size_t oSB = a.reserve((size_t)S * n * kpad);
+    const size_t oSA2 = pairs ? a.reserve((size_t)S * m * kpad) : 0;
+    const size_t oSB2 = pairs ? a.reserve((size_t)S * n * kpad) : 0;
+    char* ws;
+    TB_TRY(workspace_get(h, a.total(), &ws, 1));
+    hipStream_t st = h->stream;
+    int* eA = reinterpret_cast<int*>(ws + oEA);
+    int* eB = reinterpret_cast<int*>(ws + oEB);
+    int8_t* sA = reinterpret_cast<int8_t*>(ws + oSA);
+    int8_t* sB = reinterpret_cast<int8_t*>(ws + oSB);
+    // op(B) columns are the rows of op(B)^T
+    const int rmB = flip_mode(modeB);
+    TB_HIP(fill_i32(eA, m, kExpZero, st));
+    TB_HIP(fill_i32(eB, n, kExpZero, st));
+    if (k > 0) {
+        TB_HIP(rowexp(true, A, lda, modeA, (int)m, (int)k, 1, 0, eA, 0, st));
+        TB_HIP(rowexp(true, B, ldb, rmB, (int)n, (int)k, 1, 0, eB, 0, st));
+        if (pairs) {
+            // both products accumulate together: one scale per row and per column
+            TB_HIP(rowexp(true, A2, ldb, modeA, (int)m, (int)k, 1, 0, eA, 0, st));
+            TB_HIP(rowexp(true, B2, lda, rmB, (int)n, (int)k, 1, 0, eB, 0, st));
+        }
+    }
+    TB_HIP(ozaki_slice(A, lda, modeA, (int)m, (int)k, (int)kpad, S, 1, 0, eA, sA, st));
+    TB_HIP(ozaki_slice(B, ldb, rmB, (int)n, (int)k, (int)kpad, S, 1, 0, eB, sB, st));
+    OzakiProb p{};
+    p.sa = sA;
+    p.sb = sB;
+    if (pairs) {
+        p.sa2 = reinterpret_cast<int8_t*>(ws + oSA2);
+        p.sb2 = reinterpret_cast<int8_t*>(ws + oSB2);
+        TB_HIP(ozaki_slice(A2, ldb, modeA, (int)m, (int)k, (int)kpad, S, 1, 0, eA,
+                           const_cast<int8_t*>(p.sa2), st));
+        TB_HIP(ozaki_slice(B2, lda, rmB, (int)n, (int)k, (int)kpad, S, 1, 0, eB,
+                           const_cast<int8_t*>(p.sb2), st));
+    }
+    p.ea = eA;
+    p.eb = eB;
+    p.C = C;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.m = (int)m;
+    p.n = (int)n;
+    p.kpad = (int)kpad;
+    p.nslices = S;
+    p.kchunk = (int)ozaki_kchunk(S);
+    p.lowerC = lowerC ? 1 : 0;
+    const double flops = (lowerC ? (double)n * (n + 1) * k : 2.0 * m * n * k) * (pairs ? 2 : 1);
+    hipEvent_t e0;
+    TB_TRY(prof_begin(h, &e0));
+    TB_HIP(ozaki_gemm(p, st));
+    return prof_end(h, kProfOzakiGemm, e0, flops);
+}
+
+}  // namespace tb

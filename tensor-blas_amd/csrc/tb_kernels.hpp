@@ -1,0 +1,108 @@
+// Host-callable launchers of the library's device kernels (all asynchronous on the
+// given stream).  Kernels live in gemm_dispatch.hip, aux_kernels.hip, ozaki.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tb_common.hpp"
+
+namespace tb {
+
+// ------------------------------------------------------------- grouped GEMM
+enum GemmShape : int {
+    kShapeSquare = 0,  // 128 x 128 tiles: off-diagonal updates and whole routines
+    kShapeLeaf = 1,    // 64 x 256 tiles: a workgroup owns all n <= 256 columns of its rows,
+                       // so C may alias op(A) (in-place TRSM leaf X := B * inv(L)^T)
+};
+struct TileShape {
+    int bm, bn;
+};
+TileShape gemm_tile_shape(int tier, int shape);
+// FP16 planes per operand element of a tier: plain 1, split 3, others 0
+int tier_planes(int tier);
+// all problems of L share the operand source kinds srcA/srcB (GemmProb::srcA/srcB)
+hipError_t gemm_launch(int tier, int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s);
+
+// Pre-conversion of the rows of op(X) (FP32, rowexp modes kOpN/kOpT/kOpSymL/kOpTriL,
+// rows x K) into npl K-contiguous FP16 planes: element (r, k) of plane p at
+// out[p*sp + r*ldp + kshift + k].  npl = 1: RNE fp16(x).  npl = 3: with
+// xs = x * 2^(15 - e[r]) (e: row exponents, kExpZero -> 0), xs = h + 2^-11 m + 2^-22 l.
+// ldp must be a multiple of 64 with ldp >= kshift + K; entries of a row outside
+// [kshift, kshift + K) are written as zero.
+hipError_t split_planes(int npl, const float* X, long long ld, int mode, int rows, int K,
+                        const int* e, _Float16* out, long long ldp, long long sp, int kshift,
+                        hipStream_t s);
+
+// -------------------------------------------------------------- row exponents
+// Sentinel exponent of an all-zero row.
+constexpr int kExpZero = -100000;
+
+// out[b*so + r] := max(out[...], max_k expo(op(X)(r, k))) for r < rows, where
+// expo(x) is the e of x = f*2^e, f in [0.5, 1) (kExpZero for x == 0) and
+//   mode kOpN   : op(X)(r, k) = X[r + k*ld], k < K
+//   mode kOpTriL: same, k <= r only
+//   mode kOpSymL: row r of the symmetric matrix stored lower: X[r + k*ld] for k <= r
+//                 and X[k + r*ld] for r < k < K
+//   mode kOpT   : op(X)(r, k) = X[k + r*ld]
+// out must hold kExpZero (fill_i32) or earlier maxima.  Batched over nb items with
+// element stride sx and output stride so.
+hipError_t rowexp(bool f64, const void* X, long long ld, int mode, int rows, int K, int nb,
+                  long long sx, int* out, long long so, hipStream_t s);
+hipError_t fill_i32(int* p, long long n, int v, hipStream_t s);
+
+// ------------------------------------------------------ diagonal-block inverses
+// For q < nblocks: L_q = the s x s lower triangle at L + q*bstride (ld lda), factored
+// as L_q = (I + N) D with D = diag(L_q).  Writes Z_q = W^T with W = (I+N)^-1 - I =
+// D inv(L_q) - I (strictly lower; Z_q dense s x s, ld s, at Z + q*s*s) and
+// dinv[q*s + r] = 1 / L_q(r, r).  Then X L_q^T = B  <=>  X = (B + B W^T) D^-1.
+// FP64 arithmetic; Z is FP32 unless out_f64.  s <= 256.
+hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda,
+                         long long bstride, int s, int nblocks, void* Z, double* dinv,
+                         hipStream_t st);
+
+// ------------------------------------------------------------- conversions etc.
+// ah(i,j) = RNE fp16(as(i,j)), m x n (util/util.cu:127-135 s2h; include/TensorBLAS.h:185-284)
+hipError_t s2h(long long m, long long n, const float* as, long long ldas, void* ah, long long ldah,
+               hipStream_t s);
+// row-major m x n -> column-major m x n fp16 (util/util.cu:137-145 s2hTranspose)
+hipError_t s2h_transpose(long long m, long long n, const float* as, void* ah, hipStream_t s);
+// A(i,j) := A(j,i) for i < j (util/util.cu:148-167)
+hipError_t mirror_lower(bool f64, long long n, void* A, long long lda, hipStream_t s);
+// a(i,j) := v (util/util.cu:240-257)
+hipError_t fill_matrix(bool f64, long long m, long long n, void* a, long long lda, double v,
+                       hipStream_t s);
+// b(i,j) := a(i,j) with type conversion (util/util.cu:263-280 is the same-type case)
+hipError_t copy_matrix(bool a_f64, bool b_f64, long long m, long long n, const void* a,
+                       long long lda, void* b, long long ldb, hipStream_t s);
+// b(i,j) := a(i,j) for i >= j only (lower triangle), with type conversion
+hipError_t copy_lower(bool a_f64, bool b_f64, long long n, const void* a, long long lda, void* b,
+                      long long ldb, hipStream_t s);
+// x(i,j) += y(i,j), FP32
+hipError_t add_matrix_f32(long long m, long long n, void* x, long long ldx, const float* y,
+                          long long ldy, hipStream_t s);
+
+// ---------------------------------------------------------------------- Ozaki
+// Slices of the rows of op(X) (FP64), 7-bit digits (oracle/oracle.c orc_ozaki_slice):
+// out layout slices[(q*S + s)*rows*kpad + r*kpad + p] int8 (k padded with zeros to kpad),
+// exps[q*rows + r].  mode kOpN/kOpT/kOpSymL/kOpTriL as rowexp.  exps must be filled by
+// rowexp first.
+hipError_t ozaki_slice(const double* X, long long ld, int mode, int rows, int K, int kpad,
+                       int nslices, int nb, long long sx, const int* exps, int8_t* slices,
+                       hipStream_t s);
+struct OzakiProb {
+    const int8_t* sa;   // slices of op(A) rows   [S][m][kpad]
+    const int8_t* sb;   // slices of op(B) columns [S][n][kpad]
+    const int8_t* sa2;  // second pair (SYR2K) or null
+    const int8_t* sb2;
+    const int* ea;      // exponents of op(A) rows
+    const int* eb;      // exponents of op(B) columns
+    double* C;
+    long long ldc;
+    double alpha, beta;
+    int m, n, kpad, nslices, kchunk;
+    int lowerC;
+};
+hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s);
+
+}  // namespace tb

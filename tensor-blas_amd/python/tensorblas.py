@@ -1,0 +1,196 @@
+"""ctypes binding of the full C ABI of libTensorBLAS (include/TensorBLAS.h).
+
+Pointers are device addresses (ints), e.g. ``tensor.data_ptr()`` of a torch tensor;
+every matrix is column-major.  Every call raises ``TensorBLASError`` on a non-zero
+status.  ``torch`` is imported before the library is loaded so that the process
+keeps a single HIP runtime (torch's), which the library then shares.
+
+This module is the host-side mirror of the reference's python/TensorBLAS.py for
+all routines and tiers; ``TensorBLAS.py`` next to it keeps the reference's own
+two functions unchanged.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must be loaded first: one HIP runtime per process)
+
+_HERE = os.path.dirname(os.path.realpath(__file__))
+LIB_PATH = os.path.join(_HERE, "..", "build", "libTensorBLAS.so")
+
+TIERS = {"plain": 0, "split": 1, "ozaki": 2, "fp32": 3, "fp64": 4}
+F64_TIERS = {"ozaki", "fp64"}
+STATUS = {0: "success", 1: "invalid argument", 2: "device error", 3: "out of device memory"}
+
+
+class TensorBLASError(RuntimeError):
+    def __init__(self, fn, status):
+        super().__init__(f"{fn} failed with status {status} ({STATUS.get(status, '?')})")
+        self.status = status
+
+
+_lib = None
+_L, _D, _P, _I = ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+
+
+def lib():
+    """Load libTensorBLAS.so (raises if it is not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        path = os.path.realpath(LIB_PATH)
+        if not os.path.exists(path):
+            raise RuntimeError(f"libTensorBLAS.so is not built ({path}); run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        sig = {
+            "tb_create": [_P], "tb_destroy": [_P], "tb_set_stream": [_P, _P],
+            "tb_synchronize": [_P], "tb_set_ozaki_slices": [_P, _I],
+            "tb_syrk": [_P, _I, _L, _L, _D, _P, _L, _D, _P, _L, _L],
+            "tb_syr2k": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _L],
+            "tb_trmm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
+            "tb_trsm": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
+            "tb_trsm_refined": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
+            "tb_symm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
+            "tb_gemm": [_P, _I, _I, _I, _L, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
+            "tb_s2h": [_P, _L, _L, _P, _L, _P, _L],
+            "tb_s2h_transpose": [_P, _L, _L, _P, _P],
+            "tb_copy_lower_to_upper": [_P, _I, _L, _P, _L],
+            "tc_syrk_wrapper": [_L, _L, _P, _P, _L],
+            "tc_trsm_wrapper": [_L, _L, _P, _P, _L],
+            "tb_plan_chunks": [_I, _L, _P, _I],
+            "tb_set_profiling": [_P, _I],
+            "tb_profile_read": [_P, _P, _I],
+            "tb_plan_trsm": [_L, _L, _P, _I],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _I
+        L.tb_workspace_bytes.argtypes = [_P]
+        L.tb_workspace_bytes.restype = _L
+        L.tb_status_string.argtypes = [_I]
+        L.tb_status_string.restype = ctypes.c_char_p
+        L.tb_version.restype = _I
+        L.print_env.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(fn, st):
+    if st != 0:
+        raise TensorBLASError(fn, st)
+
+
+def tier_id(tier):
+    return TIERS[tier] if isinstance(tier, str) else int(tier)
+
+
+class Handle:
+    """A library handle (one HIP stream, one device workspace)."""
+
+    def __init__(self, stream=None):
+        self._h = ctypes.c_void_p()
+        _check("tb_create", lib().tb_create(ctypes.byref(self._h)))
+        if stream is not None:
+            self.set_stream(stream)
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def set_stream(self, stream):
+        raw = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        _check("tb_set_stream", lib().tb_set_stream(self._h, raw))
+
+    def synchronize(self):
+        _check("tb_synchronize", lib().tb_synchronize(self._h))
+
+    def set_ozaki_slices(self, s):
+        _check("tb_set_ozaki_slices", lib().tb_set_ozaki_slices(self._h, s))
+
+    PROF_NAMES = ["gemm_plain_sq", "gemm_plain_leaf", "gemm_split_sq", "gemm_split_leaf",
+                  "gemm_ozaki_sq", "gemm_ozaki_leaf", "gemm_fp32_sq", "gemm_fp32_leaf",
+                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri"]
+
+    def set_profiling(self, on):
+        _check("tb_set_profiling", lib().tb_set_profiling(self._h, int(on)))
+
+    def profile(self):
+        """{category: (launches, total_ms, flops)} for categories that ran."""
+        n = len(self.PROF_NAMES)
+        out = (ctypes.c_double * (3 * n))()
+        _check("tb_profile_read", lib().tb_profile_read(self._h, out, n))
+        return {self.PROF_NAMES[c]: (int(out[3 * c]), out[3 * c + 1], out[3 * c + 2])
+                for c in range(n) if out[3 * c] > 0}
+
+    def workspace_bytes(self):
+        return lib().tb_workspace_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            _check("tb_destroy", lib().tb_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ routines
+    def syrk(self, tier, n, k, alpha, A, lda, beta, C, ldc, nb=256):
+        _check("tb_syrk", lib().tb_syrk(self._h, tier_id(tier), n, k, alpha, A, lda, beta, C, ldc, nb))
+
+    def syr2k(self, tier, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=256):
+        _check("tb_syr2k", lib().tb_syr2k(self._h, tier_id(tier), n, k, alpha, A, lda, B, ldb, beta,
+                                          C, ldc, nb))
+
+    def trmm(self, tier, m, n, alpha, A, lda, B, ldb, C, ldc, nb=256):
+        _check("tb_trmm", lib().tb_trmm(self._h, tier_id(tier), m, n, alpha, A, lda, B, ldb, C, ldc,
+                                        nb))
+
+    def trsm(self, tier, m, n, A, lda, B, ldb, nb=256):
+        _check("tb_trsm", lib().tb_trsm(self._h, tier_id(tier), m, n, A, lda, B, ldb, nb))
+
+    def trsm_refined(self, tier, m, n, A, lda, B, ldb, nb=256):
+        _check("tb_trsm_refined", lib().tb_trsm_refined(self._h, tier_id(tier), m, n, A, lda, B, ldb,
+                                                        nb))
+
+    def symm(self, tier, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
+        _check("tb_symm", lib().tb_symm(self._h, tier_id(tier), m, n, alpha, A, lda, B, ldb, beta, C,
+                                        ldc))
+
+    def gemm(self, tier, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        _check("tb_gemm", lib().tb_gemm(self._h, tier_id(tier), int(transa), int(transb), m, n, k,
+                                        alpha, A, lda, B, ldb, beta, C, ldc))
+
+    def s2h(self, m, n, As, ldas, Ah, ldah):
+        _check("tb_s2h", lib().tb_s2h(self._h, m, n, As, ldas, Ah, ldah))
+
+    def s2h_transpose(self, m, n, As, Ah):
+        _check("tb_s2h_transpose", lib().tb_s2h_transpose(self._h, m, n, As, Ah))
+
+    def copy_lower_to_upper(self, f64, n, A, lda):
+        _check("tb_copy_lower_to_upper", lib().tb_copy_lower_to_upper(self._h, int(f64), n, A, lda))
+
+
+# --------------------------------------------------------- host-only planners
+def plan_chunks(n, trsm=False):
+    """Chunks of n as the library schedules them (util/mat_size.cu:5-55 order)."""
+    out = (ctypes.c_long * 64)()
+    c = lib().tb_plan_chunks(int(trsm), n, out, 64)
+    if c < 0:
+        raise TensorBLASError("tb_plan_chunks", -c)
+    return list(out[:c])
+
+
+def plan_trsm(n, nb):
+    """TRSM schedule: list of ('leaf', off, s) and ('update', c0, w, k0, kw) in order."""
+    cap = 1 << 16
+    out = (ctypes.c_long * (5 * cap))()
+    c = lib().tb_plan_trsm(n, nb, out, cap)
+    if c < 0:
+        raise TensorBLASError("tb_plan_trsm", -c)
+    ops = []
+    for q in range(c):
+        t, a, b, d, e = out[5 * q:5 * q + 5]
+        ops.append(("leaf", a, b) if t == 0 else ("update", a, b, d, e))
+    return ops
