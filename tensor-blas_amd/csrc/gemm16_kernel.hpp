@@ -1,28 +1,31 @@
 // Grouped FP16-MFMA GEMM for the plain and split tiers (gfx950, v_mfma_f32_32x32x16_f16).
 //
 // C = alpha * sum_pairs op(A) op(B) + beta * C over a flat list of output tiles of up to
-// kMaxProbs strided-batch problems (tb_common.hpp).  512 threads (8 waves) own one
-// BM x BN tile; K runs in BK steps through two LDS buffers: the global loads of step
-// s+1 are in flight while the MFMAs of step s run.
+// kMaxProbs strided-batch problems (tb_common.hpp).  256 threads (4 waves, 64 x 64
+// accumulator tile each) own one 128 x 128 tile (64 x 256 for TRSM leaves) and walk K
+// in BK = 32 steps through a ring of NST LDS stages.
 //
-// Each operand arrives in one of two forms (template SRCA / SRCB):
-//   kSrcF32    FP32 matrix in any of the kOp* layouts, converted in registers on its way
-//              to LDS (used for operands that change during a routine, e.g. TRSM's
-//              solved block X);
-//   kSrcPlanes NPL pre-converted FP16 planes laid out K-contiguous (plane[r*ldp + k]),
-//              copied to LDS with 16-byte loads and no arithmetic.
+// Operands arrive in one of two forms (template SRCA / SRCB):
+//   kSrcPlanes NPL pre-converted FP16 planes, K-contiguous (plane[r*ldp + k]).  They are
+//              copied global -> LDS by the CDNA4 direct-to-LDS loads
+//              (global_load_lds_dwordx4): no registers, no ds_write, and each stage's
+//              loads are issued NST-1 stages ahead of their use.
+//   kSrcF32    an FP32 matrix in any of the kOp* layouts, loaded one stage ahead into
+//              registers and converted on its way to LDS (operands that change during
+//              a routine, e.g. TRSM's solved block X).
 // NPL = 1 (plain tier): x -> RNE fp16(x), the reference's s2h conversion.
 // NPL = 3 (split tier): xs = x * 2^(15 - e_row) is written exactly as
 //   xs = h + 2^-11 m + 2^-22 l (h, m, l fp16; 33 significant bits >= FP32's 24)
-// and the six products of total weight >= 2^-22 are accumulated in three FP32
-// accumulators:  acc0 += hh;  acc1 += hm + mh;  acc2 += hl + mm + lh.
-// C = (acc0 + 2^-11 (acc1 + 2^-11 acc2)) * 2^-(scA + scB): products whose weight is
-// 2^-33 or less are dropped, so the result carries FP32 accumulation error only.
+// and the six products of total weight >= 2^-22 go to three FP32 accumulators:
+//   acc0 += hh;  acc1 += hm + mh;  acc2 += hl + mm + lh;
+// C = (acc0 + 2^-11 (acc1 + 2^-11 acc2)) * 2^-(scA + scB).
 //
-// LDS holds each plane of a panel as [row][BK + 8] halves: every MFMA fragment is one
-// 16-byte ds_read and rows are 80 bytes apart (bank-conflict free for 8 lanes).  The
-// MFMA's A input comes from the op(B) panel and its B input from the op(A) panel, so
-// the accumulator's lane index runs along C's rows and column-major stores coalesce.
+// LDS: every plane of a panel is [row][32 halves] = 64-byte rows, unpadded; the four
+// 16-byte units of row r sit at unit position u ^ ((r >> 2) & 3).  Both the 16-lane
+// store pattern (4 rows x 4 units) and the MFMA fragment reads (16 rows, one unit) then
+// cover all 64 banks once.  The MFMA's A input comes from the op(B) panel and its B
+// input from the op(A) panel, so the accumulator's lane index runs along C's rows and
+// the column-major stores of C coalesce.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,188 +34,236 @@
 
 namespace tb {
 
-// One operand panel: R rows (of op(A), or columns of op(B)) x BK, staged by NT threads.
-template <int NPL, int SRC, int R, int BK, int NT>
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+constexpr int kBK = 32;          // k per stage
+constexpr int kRowBytes = 64;    // one LDS row: 32 halves
+constexpr int kGroupM = 8;       // tile rows per group in the tile order of full problems
+
+__device__ __forceinline__ int swz(int row, int unit) { return unit ^ ((row >> 2) & 3); }
+
+// "s_waitcnt vmcnt(n)" then a workgroup barrier.  Written as one asm statement with a
+// memory clobber: the compiler neither moves memory operations across it nor adds the
+// vmcnt(0) a fenced __syncthreads() would (which would also wait for the prefetched
+// stages still in flight).  lgkmcnt(0) completes this wave's ds_writes first.
+#define TB_WAIT_VM_BARRIER(n) \
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"i"(n) : "memory")
+
+// One direct-to-LDS load: 16 bytes from g (per lane) to LDS[lds_base + 16*lane].
+// Issued as asm so that the compiler does not guard every later LDS read with a
+// vmcnt(0) for it (it cannot tell the ring slots apart); the kernel's own
+// TB_WAIT_VM_BARRIER accounting orders it.  The instruction has no result register;
+// its address registers are consumed at issue.
+__device__ __forceinline__ void lds_dma16(const void* g, unsigned lds_base)
+{
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds_base), "v"(g)
+                 : "memory", "m0");
+}
+
+// One operand panel of R rows x kBK with NPL planes.
+template <int NPL, int SRC, int R, int NT>
 struct Panel16 {
-    static constexpr int KPT = 8;                 // k per work item (one 16-byte plane row)
-    static constexpr int TPR = BK / KPT;          // items per panel row
-    static constexpr int ITEMS_TOTAL = R * TPR;
-    static constexpr int ITEMS = (ITEMS_TOTAL + NT - 1) / NT;
-    static constexpr int LDK = BK + 8;            // LDS row stride (halves)
-    static constexpr int PLANE = R * LDK;         // halves per plane
-    static_assert(BK % KPT == 0, "BK");
-    static_assert(ITEMS_TOTAL % NT == 0 || ITEMS_TOTAL < NT, "items");
+    static constexpr int UNITS = R * 4;                 // 16-byte units per plane
+    static constexpr int PLANE_BYTES = R * kRowBytes;
+    static constexpr int BYTES = NPL * PLANE_BYTES;
+    // direct-to-LDS: one instruction moves 64 units (1 KB) per wave
+    static constexpr int DMA_PER_PLANE = UNITS / 64;    // wave-instructions per plane
+    static constexpr int NWAVES = NT / 64;
+    static constexpr int DMA_PER_WAVE = (NPL * DMA_PER_PLANE + NWAVES - 1) / NWAVES;
+    // register staging (kSrcF32): ITEMS units of 8 k per thread
+    static constexpr int ITEMS = (UNITS + NT - 1) / NT;
+    static_assert(UNITS % 64 == 0, "rows");
+    // every wave issues the same number of direct-to-LDS loads per stage, so one
+    // vmcnt value describes a stage for all of them
+    static_assert((NPL * DMA_PER_PLANE) % NWAVES == 0, "even DMA split");
 
-    // staging registers
-    float vf[SRC == kSrcF32 ? ITEMS : 1][KPT];
-    half8_t vh[SRC == kSrcPlanes ? ITEMS : 1][NPL];
-    int row[ITEMS], kseg[ITEMS];
-    bool live[ITEMS];
-    int sc[ITEMS];   // split: scale exponent of the item's row
+    float vf[SRC == kSrcF32 ? ITEMS : 1][8];
+    int row[SRC == kSrcF32 ? ITEMS : 1], unit[SRC == kSrcF32 ? ITEMS : 1];
+    bool live[SRC == kSrcF32 ? ITEMS : 1];
+    int sc[SRC == kSrcF32 ? ITEMS : 1];
 
-    __device__ __forceinline__ void setup(int tid, bool mn_major)
+    // ---------------------------------------------------------------- planes
+    // This wave's share of a stage's direct-to-LDS loads: instruction q moves 16 plane
+    // rows (1 KB).  gp[q] is this lane's source at k = 0 (rows past `rows` clamped onto
+    // row rows-1: their C entries are never stored; k past the problem's K must hold
+    // zeros in the planes, which the host guarantees), ld[q] its LDS byte offset.
+    const _Float16* gp[SRC == kSrcPlanes ? DMA_PER_WAVE : 1];
+    unsigned ldo[SRC == kSrcPlanes ? DMA_PER_WAVE : 1];
+
+    __device__ __forceinline__ void dma_setup(const _Float16* __restrict__ P, long long ldp,
+                                             long long sp, int r0, int rows, int wave, int lane)
     {
+        if constexpr (SRC == kSrcPlanes) {
 #pragma unroll
-        for (int q = 0; q < ITEMS; ++q) {
-            const int it = tid + q * NT;
-            live[q] = it < ITEMS_TOTAL;
-            if (mn_major) {
-                row[q] = it % R;
-                kseg[q] = it / R;
-            } else {
-                row[q] = it / TPR;
-                kseg[q] = it % TPR;
+            for (int q = 0; q < DMA_PER_WAVE; ++q) {
+                const int inst = wave + q * NWAVES;          // 0 .. NPL*DMA_PER_PLANE-1
+                const int p = inst / DMA_PER_PLANE;
+                const int blk = inst % DMA_PER_PLANE;
+                const int pos = blk * 64 + lane;            // physical unit in the plane
+                const int rl = pos >> 2;
+                const int u = swz(rl, pos & 3);             // logical unit stored there
+                int r = r0 + rl;
+                r = r < rows ? r : rows - 1;
+                gp[q] = P + p * sp + (long long)r * ldp + u * 8;
+                ldo[q] = (unsigned)(p * PLANE_BYTES + blk * 1024);
             }
-            sc[q] = 0;
         }
     }
 
-    // FP32 source, op(X) rows [r0, r0+R) x k [k0, k0+BK); layouts as in gemm_kernel.hpp
+    __device__ __forceinline__ void dma(int k0, unsigned lds_base) const
+    {
+        if constexpr (SRC == kSrcPlanes) {
+#pragma unroll
+            for (int q = 0; q < DMA_PER_WAVE; ++q) lds_dma16(gp[q] + k0, lds_base + ldo[q]);
+        }
+    }
+
+    // ------------------------------------------------------------- FP32 source
+    __device__ __forceinline__ void setup(int tid, bool mn_major)
+    {
+        if constexpr (SRC == kSrcF32) {
+#pragma unroll
+            for (int q = 0; q < ITEMS; ++q) {
+                const int it = tid + q * NT;
+                live[q] = it < UNITS;
+                if (mn_major) {            // consecutive threads: consecutive rows
+                    row[q] = it % R;
+                    unit[q] = it / R;
+                } else {                   // consecutive threads: consecutive k
+                    row[q] = it >> 2;
+                    unit[q] = it & 3;
+                }
+                sc[q] = 0;
+            }
+        }
+    }
+
     __device__ __forceinline__ void load_f32(const float* __restrict__ X, long long ld, int mode,
                                              bool isA, int r0, int k0, int rows, int K,
                                              bool vec_ok)
     {
-        int m = mode;
-        bool zero = false, special = false;
-        if (isA && (mode == kOpSymL || mode == kOpTriL)) {
-            if (k0 + BK <= r0) m = kOpN;
-            else if (k0 >= r0 + R) { m = kOpT; zero = (mode == kOpTriL); }
-            else special = true;
-        }
-        const bool mn = isA ? (m != kOpT) : (m == kOpT);
-        const bool full = (r0 + R <= rows) && (k0 + BK <= K);
-#pragma unroll
-        for (int q = 0; q < ITEMS; ++q) {
-            const int r = r0 + row[q];
-            const int kb = k0 + kseg[q] * KPT;
-            if (zero || !live[q]) {
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) vf[q][j] = 0.0f;
-                continue;
+        if constexpr (SRC == kSrcF32) {
+            int m = mode;
+            bool zero = false, special = false;
+            if (isA && (mode == kOpSymL || mode == kOpTriL)) {
+                if (k0 + kBK <= r0) m = kOpN;
+                else if (k0 >= r0 + R) { m = kOpT; zero = (mode == kOpTriL); }
+                else special = true;
             }
-            if (!special && full) {
-                if (mn) {
-                    const float* p = X + (long long)r + (long long)kb * ld;
+            const bool mn = isA ? (m != kOpT) : (m == kOpT);
+            const bool full = (r0 + R <= rows) && (k0 + kBK <= K);
 #pragma unroll
-                    for (int j = 0; j < KPT; ++j) vf[q][j] = p[(long long)j * ld];
-                } else if (vec_ok) {
-                    const float* p = X + (long long)r * ld + kb;
-                    const float4_t a = *reinterpret_cast<const float4_t*>(p);
-                    const float4_t b = *reinterpret_cast<const float4_t*>(p + 4);
-                    vf[q][0] = a[0]; vf[q][1] = a[1]; vf[q][2] = a[2]; vf[q][3] = a[3];
-                    vf[q][4] = b[0]; vf[q][5] = b[1]; vf[q][6] = b[2]; vf[q][7] = b[3];
-                } else {
-                    const float* p = X + (long long)r * ld + kb;
+            for (int q = 0; q < ITEMS; ++q) {
+                const int r = r0 + row[q];
+                const int kb = k0 + unit[q] * 8;
+                if (zero || !live[q]) {
 #pragma unroll
-                    for (int j = 0; j < KPT; ++j) vf[q][j] = p[j];
+                    for (int j = 0; j < 8; ++j) vf[q][j] = 0.0f;
+                    continue;
                 }
-                continue;
-            }
+                if (!special && full) {
+                    if (mn) {
+                        const float* p = X + (long long)r + (long long)kb * ld;
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const int k = kb + j;
-                float x = 0.0f;
-                if (r < rows && k < K) {
-                    if (special) {
-                        if (r >= k) x = X[(long long)r + (long long)k * ld];
-                        else if (mode == kOpSymL) x = X[(long long)k + (long long)r * ld];
-                    } else if (mn) {
-                        x = X[(long long)r + (long long)k * ld];
+                        for (int j = 0; j < 8; ++j) vf[q][j] = p[(long long)j * ld];
+                    } else if (vec_ok) {
+                        const float* p = X + (long long)r * ld + kb;
+                        const float4_t a = *reinterpret_cast<const float4_t*>(p);
+                        const float4_t b = *reinterpret_cast<const float4_t*>(p + 4);
+                        vf[q][0] = a[0]; vf[q][1] = a[1]; vf[q][2] = a[2]; vf[q][3] = a[3];
+                        vf[q][4] = b[0]; vf[q][5] = b[1]; vf[q][6] = b[2]; vf[q][7] = b[3];
                     } else {
-                        x = X[(long long)r * ld + k];
+                        const float* p = X + (long long)r * ld + kb;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) vf[q][j] = p[j];
                     }
+                    continue;
                 }
-                vf[q][j] = x;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = kb + j;
+                    float x = 0.0f;
+                    if (r < rows && k < K) {
+                        if (special) {
+                            if (r >= k) x = X[(long long)r + (long long)k * ld];
+                            else if (mode == kOpSymL) x = X[(long long)k + (long long)r * ld];
+                        } else if (mn) {
+                            x = X[(long long)r + (long long)k * ld];
+                        } else {
+                            x = X[(long long)r * ld + k];
+                        }
+                    }
+                    vf[q][j] = x;
+                }
             }
         }
     }
 
-    // pre-converted planes: plane p of row r at P + p*sp + r*ldp (K-contiguous, zero
-    // padded up to a multiple of 8 in k)
-    __device__ __forceinline__ void load_planes(const _Float16* __restrict__ P, long long ldp,
-                                                long long sp, int r0, int k0, int rows, int K)
+    __device__ __forceinline__ void store_f32(char* lds) const
     {
+        if constexpr (SRC == kSrcF32) {
 #pragma unroll
-        for (int q = 0; q < ITEMS; ++q) {
-            const int r = r0 + row[q];
-            const int kb = k0 + kseg[q] * KPT;
-            const bool ok = live[q] && r < rows && kb < K;
+            for (int q = 0; q < ITEMS; ++q) {
+                if (!live[q]) continue;
+                char* dst = lds + row[q] * kRowBytes + swz(row[q], unit[q]) * 16;
+                if constexpr (NPL == 1) {
+                    half8_t hh;
 #pragma unroll
-            for (int p = 0; p < NPL; ++p) {
-                if (ok)
-                    vh[q][p] = *reinterpret_cast<const half8_t*>(P + p * sp + (long long)r * ldp + kb);
-                else
-                    vh[q][p] = half8_t{};
-            }
-            // planes may continue past this problem's K (a sub-block of a larger matrix)
-            if (ok && kb + KPT > K) {
+                    for (int j = 0; j < 8; ++j) hh[j] = (_Float16)vf[q][j];
+                    *reinterpret_cast<half8_t*>(dst) = hh;
+                } else {
+                    half8_t hh, mm, ll;
 #pragma unroll
-                for (int j = 0; j < KPT; ++j)
-                    if (kb + j >= K)
-#pragma unroll
-                        for (int p = 0; p < NPL; ++p) vh[q][p][j] = (_Float16)0.0f;
-            }
-        }
-    }
-
-    __device__ __forceinline__ void store(_Float16* base) const
-    {
-#pragma unroll
-        for (int q = 0; q < ITEMS; ++q) {
-            if (!live[q]) continue;
-            _Float16* dst = base + row[q] * LDK + kseg[q] * KPT;
-            if constexpr (SRC == kSrcPlanes) {
-#pragma unroll
-                for (int p = 0; p < NPL; ++p) *reinterpret_cast<half8_t*>(dst + p * PLANE) = vh[q][p];
-            } else if constexpr (NPL == 1) {
-                half8_t hh;
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) hh[j] = (_Float16)vf[q][j];
-                *reinterpret_cast<half8_t*>(dst) = hh;
-            } else {
-                half8_t hh, mm, ll;
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) {
-                    const float xs = __builtin_ldexpf(vf[q][j], sc[q]);
-                    const _Float16 h = (_Float16)xs;
-                    const float r1 = (xs - (float)h) * 2048.0f;      // exact
-                    const _Float16 m = (_Float16)r1;
-                    const float r2 = (r1 - (float)m) * 2048.0f;      // exact
-                    hh[j] = h;
-                    mm[j] = m;
-                    ll[j] = (_Float16)r2;
+                    for (int j = 0; j < 8; ++j) {
+                        const float xs = __builtin_ldexpf(vf[q][j], sc[q]);
+                        const _Float16 h = (_Float16)xs;
+                        const float r1 = (xs - (float)h) * 2048.0f;      // exact
+                        const _Float16 m = (_Float16)r1;
+                        const float r2 = (r1 - (float)m) * 2048.0f;      // exact
+                        hh[j] = h;
+                        mm[j] = m;
+                        ll[j] = (_Float16)r2;
+                    }
+                    *reinterpret_cast<half8_t*>(dst) = hh;
+                    *reinterpret_cast<half8_t*>(dst + PLANE_BYTES) = mm;
+                    *reinterpret_cast<half8_t*>(dst + 2 * PLANE_BYTES) = ll;
                 }
-                *reinterpret_cast<half8_t*>(dst) = hh;
-                *reinterpret_cast<half8_t*>(dst + PLANE) = mm;
-                *reinterpret_cast<half8_t*>(dst + 2 * PLANE) = ll;
             }
         }
     }
 };
 
-// tile order inside a full problem: groups of kGroupM tile rows, column by column,
-// so the ~32 tiles an XCD runs at once share 8 row panels and 4 column panels
-constexpr int kGroupM = 8;
-
-template <int NPL, int SRCA, int SRCB, int BM, int BN, int BK, int WM, int WN>
+template <int NPL, int SRCA, int SRCB, int BM, int BN, int WM, int WN, int NST>
 __global__ void __launch_bounds__(64 * WM * WN)
 gemm16_kernel(GemmLaunch L)
 {
     constexpr int NT = 64 * WM * WN;
-    using PA = Panel16<NPL, SRCA, BM, BK, NT>;
-    using PB = Panel16<NPL, SRCB, BN, BK, NT>;
+    using PA = Panel16<NPL, SRCA, BM, NT>;
+    using PB = Panel16<NPL, SRCB, BN, NT>;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 32, FN = WTN / 32;
     static_assert(FM >= 1 && FN >= 1, "wave tile");
-    constexpr int STAGE = NPL * (PA::PLANE + PB::PLANE);
+    constexpr int STAGE = PA::BYTES + PB::BYTES;
     constexpr bool kSplit = (NPL == 3);
+    // direct-to-LDS instructions each wave issues per stage (the vmcnt of one stage)
+    constexpr int DMA_W = (SRCA == kSrcPlanes ? PA::DMA_PER_WAVE : 0) +
+                          (SRCB == kSrcPlanes ? PB::DMA_PER_WAVE : 0);
+    static_assert(NST >= 2 && NST <= 3, "stages");
+    // loads that may stay in flight when stage s+1 must be complete: those of stages
+    // s+2 .. s+NST-1 (issued later, completing later: vmcnt retires in order)
+    constexpr int KEEP = (NST - 2) * DMA_W;
+    static_assert(KEEP <= 60, "vmcnt");
 
-    __shared__ __attribute__((aligned(16))) _Float16 smem[2 * STAGE];
+    __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
     __shared__ int scA_s[kSplit ? BM : 1];
     __shared__ int scB_s[kSplit ? BN : 1];
     __shared__ int emax_s[BM];
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (SGPR)
     const int g = xcd_remap(blockIdx.x, L.totalTiles);
     int pi = 0;
 #pragma unroll 1
@@ -243,11 +294,10 @@ gemm16_kernel(GemmLaunch L)
 
     int kend = K;
     if (P.modeA == kOpTriL) kend = min(K, i0 + BM);
-    const int kblocks = (kend + BK - 1) / BK;
+    const int kblocks = (kend + kBK - 1) / kBK;
     const bool pairs = P.A2 != nullptr;
     const int nst = kblocks * (pairs ? 2 : 1);
 
-    // operand bases of this batch element
     const float* Af = reinterpret_cast<const float*>(P.A) + (long long)bi * P.sA;
     const float* A2f = pairs ? reinterpret_cast<const float*>(P.A2) + (long long)bi * P.sA : nullptr;
     const float* Bf = reinterpret_cast<const float*>(P.B) + (long long)bi * P.sB;
@@ -260,8 +310,8 @@ gemm16_kernel(GemmLaunch L)
 
     PA pa;
     PB pb;
-    pa.setup(tid, SRCA == kSrcF32 && P.modeA != kOpT);
-    pb.setup(tid, SRCB == kSrcF32 && P.modeB == kOpT);
+    pa.setup(tid, P.modeA != kOpT);
+    pb.setup(tid, P.modeB == kOpT);
     const bool vecA = SRCA == kSrcF32 &&
                       ((reinterpret_cast<uintptr_t>(Af) | reinterpret_cast<uintptr_t>(A2f)) % 16 == 0) &&
                       (P.lda % 4 == 0);
@@ -278,35 +328,51 @@ gemm16_kernel(GemmLaunch L)
     }
     for (int r = tid; r < BM; r += NT) emax_s[r] = kExpZeroDev;
     __syncthreads();
-    if constexpr (kSplit) {
+    if constexpr (kSplit && SRCA == kSrcF32) {
 #pragma unroll
         for (int q = 0; q < PA::ITEMS; ++q) pa.sc[q] = scA_s[pa.row[q]];
+    }
+    if constexpr (kSplit && SRCB == kSrcF32) {
 #pragma unroll
         for (int q = 0; q < PB::ITEMS; ++q) pb.sc[q] = scB_s[pb.row[q]];
     }
 
-    auto stage_load = [&](int s) {
-        const int pair = s >= kblocks;
-        const int k0 = (s - pair * kblocks) * BK;
+    auto stage_k0 = [&](int s, int& pair) {
+        pair = s >= kblocks;
+        return (s - pair * kblocks) * kBK;
+    };
+    // FP32 operands of stage s into registers
+    auto load_regs = [&](int s) {
+        int pair;
+        const int k0 = stage_k0(s, pair);
         if constexpr (SRCA == kSrcF32)
             pa.load_f32(pair ? A2f : Af, P.lda, P.modeA, true, i0, k0, M, K, vecA);
-        else
-            pa.load_planes(pair ? A2h : Ah, P.lda, P.spA, i0, k0, M, K);
         if constexpr (SRCB == kSrcF32)
             pb.load_f32(pair ? B2f : Bf, P.ldb, P.modeB, false, j0, k0, N, K, vecB);
-        else
-            pb.load_planes(pair ? B2h : Bh, P.ldb, P.spB, j0, k0, N, K);
     };
-    auto stage_store = [&](int buf) {
-        _Float16* base = smem + buf * STAGE;
-        pa.store(base);
-        pb.store(base + NPL * PA::PLANE);
+    auto store_regs = [&](int s) {
+        char* base = smem + (s % NST) * STAGE;
+        pa.store_f32(base);
+        pb.store_f32(base + PA::BYTES);
+    };
+    // plane operands of stage s straight into LDS (sources re-derived when the second
+    // product pair starts)
+    const unsigned smem_lds = (unsigned)(uintptr_t)(lds_void_t*)smem;
+    int dma_pair = -1;
+    auto issue_dma = [&](int s) {
+        int pair;
+        const int k0 = stage_k0(s, pair);
+        if (pair != dma_pair) {
+            pa.dma_setup(pair ? A2h : Ah, P.lda, P.spA, i0, M, wave, lane);
+            pb.dma_setup(pair ? B2h : Bh, P.ldb, P.spB, j0, N, wave, lane);
+            dma_pair = pair;
+        }
+        const unsigned base = smem_lds + (unsigned)((s % NST) * STAGE);
+        pa.dma(k0, base);
+        pb.dma(k0, base + PA::BYTES);
     };
 
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
-
     constexpr int NACC = FM * FN;
     floatx16_t acc0[NACC];
     floatx16_t acc1[kSplit ? NACC : 1];
@@ -319,62 +385,88 @@ gemm16_kernel(GemmLaunch L)
         acc2[q] = floatx16_t{};
     }
 
-    auto compute = [&](int buf) {
-        const _Float16* sA = smem + buf * STAGE;
-        const _Float16* sB = sA + NPL * PA::PLANE;
-        const int fr = lane & 31, fk = (lane >> 5) * 8;
+    // fragment addressing: lane reads row (fragment base + lane%32), 16-byte unit
+    // kk/8 + lane/32, swizzled by bits 2..3 of the row (the base is a multiple of 32)
+    const int frow = lane & 31, fsw = (frow >> 2) & 3, fu = lane >> 5;
+    // fragments of the 16-wide k half `kk` of stage s, then their MFMAs
+    struct Frags {
+        half8_t a[FM][NPL], b[FN][NPL];
+    };
+    auto load_frags = [&](int s, int kk, Frags& f) {
+        const char* sA = smem + (s % NST) * STAGE;
+        const char* sB = sA + PA::BYTES;
+        const int uo = ((kk >> 3) + fu) ^ fsw;
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            half8_t af[FM][NPL], bf[FN][NPL];
+        for (int a = 0; a < FM; ++a) {
+            const int off = (wm * WTM + a * 32 + frow) * kRowBytes + uo * 16;
 #pragma unroll
-            for (int a = 0; a < FM; ++a) {
-                const int off = (wm * WTM + a * 32 + fr) * PA::LDK + kk + fk;
+            for (int p = 0; p < NPL; ++p)
+                f.a[a][p] = *reinterpret_cast<const half8_t*>(sA + p * PA::PLANE_BYTES + off);
+        }
 #pragma unroll
-                for (int p = 0; p < NPL; ++p)
-                    af[a][p] = *reinterpret_cast<const half8_t*>(sA + p * PA::PLANE + off);
-            }
+        for (int b = 0; b < FN; ++b) {
+            const int off = (wn * WTN + b * 32 + frow) * kRowBytes + uo * 16;
 #pragma unroll
-            for (int b = 0; b < FN; ++b) {
-                const int off = (wn * WTN + b * 32 + fr) * PB::LDK + kk + fk;
+            for (int p = 0; p < NPL; ++p)
+                f.b[b][p] = *reinterpret_cast<const half8_t*>(sB + p * PB::PLANE_BYTES + off);
+        }
+    };
+    // dependent MFMAs on one accumulator are NACC issues apart
+    auto mma = [&](const Frags& f) {
 #pragma unroll
-                for (int p = 0; p < NPL; ++p)
-                    bf[b][p] = *reinterpret_cast<const half8_t*>(sB + p * PB::PLANE + off);
-            }
-            // products ordered so dependent MFMAs on one accumulator are >= NACC apart
+        for (int q = 0; q < NACC; ++q)
+            acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][0], acc0[q], 0, 0, 0);
+        if constexpr (kSplit) {
 #pragma unroll
             for (int q = 0; q < NACC; ++q)
-                acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(bf[q / FM][0], af[q % FM][0], acc0[q], 0, 0, 0);
-            if constexpr (kSplit) {
+                acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][1], acc1[q], 0, 0, 0);
 #pragma unroll
-                for (int q = 0; q < NACC; ++q)
-                    acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(bf[q / FM][0], af[q % FM][1], acc1[q], 0, 0, 0);
+            for (int q = 0; q < NACC; ++q)
+                acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][2], acc2[q], 0, 0, 0);
 #pragma unroll
-                for (int q = 0; q < NACC; ++q)
-                    acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(bf[q / FM][0], af[q % FM][2], acc2[q], 0, 0, 0);
+            for (int q = 0; q < NACC; ++q)
+                acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][1], f.a[q % FM][0], acc1[q], 0, 0, 0);
 #pragma unroll
-                for (int q = 0; q < NACC; ++q)
-                    acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(bf[q / FM][1], af[q % FM][0], acc1[q], 0, 0, 0);
+            for (int q = 0; q < NACC; ++q)
+                acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][1], f.a[q % FM][1], acc2[q], 0, 0, 0);
 #pragma unroll
-                for (int q = 0; q < NACC; ++q)
-                    acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(bf[q / FM][1], af[q % FM][1], acc2[q], 0, 0, 0);
-#pragma unroll
-                for (int q = 0; q < NACC; ++q)
-                    acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(bf[q / FM][2], af[q % FM][0], acc2[q], 0, 0, 0);
-            }
+            for (int q = 0; q < NACC; ++q)
+                acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][2], f.a[q % FM][0], acc2[q], 0, 0, 0);
         }
     };
 
+    // ------------------------------------------------------------------ pipeline
+    // Stage s lives in LDS slot s % NST.  Iteration s: FP32 loads of s+1 (registers),
+    // direct-to-LDS loads of s+NST-1, MFMAs of s, FP32 stores of s+1, then wait until
+    // only the loads of stage s+NST-1 are outstanding (vmcnt(DMA_W)) and barrier: stage
+    // s+1 is complete in LDS for every wave, and slot s % NST is free to be reloaded.
     if (nst > 0) {
-        stage_load(0);
-        stage_store(0);
-        __syncthreads();
+        if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32) load_regs(0);
+#pragma unroll
+        for (int s = 0; s < NST - 1; ++s)
+            if (s < nst) issue_dma(s);
+        if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32) store_regs(0);
+        // stage 0 complete: only stages 1..NST-2 may still be in flight
+        if (KEEP > 0 && nst > 1) TB_WAIT_VM_BARRIER(KEEP);
+        else TB_WAIT_VM_BARRIER(0);
 #pragma unroll 1
         for (int s = 0; s < nst; ++s) {
-            const bool more = (s + 1 < nst);
-            if (more) stage_load(s + 1);
-            compute(s & 1);
-            if (more) stage_store((s + 1) & 1);
-            __syncthreads();
+            const bool more = s + 1 < nst;
+            if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32)
+                if (more) load_regs(s + 1);
+            const bool ahead = s + NST - 1 < nst;
+            // both halves' fragment reads go out before the first MFMA; the next
+            // stage's loads are issued between the halves, overlapping the MFMAs
+            Frags f0, f1;
+            load_frags(s, 0, f0);
+            load_frags(s, 16, f1);
+            mma(f0);
+            if (ahead) issue_dma(s + NST - 1);
+            mma(f1);
+            if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32)
+                if (more) store_regs(s + 1);
+            if (KEEP > 0 && ahead) TB_WAIT_VM_BARRIER(KEEP);
+            else TB_WAIT_VM_BARRIER(0);
         }
     }
 

@@ -15,10 +15,10 @@ hipError_t run(const GemmLaunch& L, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int NPL, int SA, int SB, int BM, int BN, int BK, int WM, int WN>
+template <int NPL, int SA, int SB, int BM, int BN, int WM, int WN, int NST>
 hipError_t run16(const GemmLaunch& L, hipStream_t s)
 {
-    hipLaunchKernelGGL((gemm16_kernel<NPL, SA, SB, BM, BN, BK, WM, WN>), dim3(L.totalTiles),
+    hipLaunchKernelGGL((gemm16_kernel<NPL, SA, SB, BM, BN, WM, WN, NST>), dim3(L.totalTiles),
                        dim3(64 * WM * WN), 0, s, L);
     return hipGetLastError();
 }
@@ -26,8 +26,10 @@ hipError_t run16(const GemmLaunch& L, hipStream_t s)
 template <int NPL, int SA, int SB>
 hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s)
 {
-    return shape == kShapeLeaf ? run16<NPL, SA, SB, 64, 256, 32, 2, 4>(L, s)
-                               : run16<NPL, SA, SB, 128, 128, 32, 4, 2>(L, s);
+    // four waves, 64 x 64 accumulator tile per wave; three LDS stages where they fit
+    // in 160 KB (the split leaf tile holds two)
+    return shape == kShapeLeaf ? run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s)
+                               : run16<NPL, SA, SB, 128, 128, 2, 2, 3>(L, s);
 }
 
 template <int NPL>

@@ -474,8 +474,10 @@ struct TrsmCtx {
                                    (double*)Bcol(c0), ldb, false);
         GemmProb p = make_prob(m, w, kw, Bcol(k0), ldb, kOpN, Aat(c0, k0), lda, kOpT, Bcol(c0), ldb,
                                -1.0, 1.0, kCFull);
-        // the L planes serve updates of the update tier whose column offset is aligned
-        if (l_planes() && tier == upd_tier && (k0 + kshift) % 8 == 0) {
+        // the L planes serve updates of the update tier whose column offset is 16-byte
+        // aligned and whose width is whole 32-column stages (the kernel copies planes
+        // by stages and the planes continue past the block)
+        if (l_planes() && tier == upd_tier && (k0 + kshift) % 8 == 0 && kw % 32 == 0) {
             p.B = Lp.p + c0 * Lp.ldp + kshift + k0;
             p.ldb = Lp.ldp;
             p.spB = Lp.sp;
