@@ -198,12 +198,14 @@ __global__ void add_matrix_f32_kernel(long long m, long long n, float* x, long l
 }
 
 // 64 rows x 64 plane columns per block through an LDS tile, so both the source read
-// (rows or k contiguous) and the 16-byte plane writes are coalesced.
+// (rows or k contiguous) and the 16-byte plane writes are coalesced.  e_used (split
+// tier, optional) receives the row exponents the planes were scaled with, for a GEMM
+// that runs while e itself keeps growing.
 template <int NPL>
 __global__ void __launch_bounds__(256)
 split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int rows, int K,
-                    const int* __restrict__ e, _Float16* __restrict__ out, long long ldp,
-                    long long sp, int kshift)
+                    const int* __restrict__ e, int* __restrict__ e_used,
+                    _Float16* __restrict__ out, long long ldp, long long sp, int kshift)
 {
     __shared__ float t[64][65];
     const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;   // c: plane column = kshift + k
@@ -230,6 +232,7 @@ split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int row
     if (NPL == 3) {
         const int ex = e[r];
         sc = 15 - (ex < -1000 ? 0 : (ex > 1000 ? 1000 : ex));
+        if (e_used && blockIdx.x == 0 && (tid & 3) == 0) e_used[r] = ex;
     }
     typedef _Float16 half8 __attribute__((ext_vector_type(8)));
     for (int half = 0; half < 2; ++half) {
@@ -273,7 +276,7 @@ dim3 grid2d(long long m, long long n)
 
 hipError_t split_planes(int npl, const float* X, long long ld, int mode, int rows, int K,
                         const int* e, _Float16* out, long long ldp, long long sp, int kshift,
-                        hipStream_t s)
+                        hipStream_t s, int* e_used)
 {
     if (rows <= 0) return hipSuccess;
     if (ldp % 64 || kshift < 0 || ldp < kshift + K || (npl != 1 && npl != 3) ||
@@ -281,11 +284,11 @@ hipError_t split_planes(int npl, const float* X, long long ld, int mode, int row
         return hipErrorInvalidValue;
     dim3 g((unsigned)(ldp / 64), (unsigned)((rows + 63) / 64));
     if (npl == 1)
-        hipLaunchKernelGGL(split_planes_kernel<1>, g, dim3(256), 0, s, X, ld, mode, rows, K, e, out,
-                           ldp, sp, kshift);
+        hipLaunchKernelGGL(split_planes_kernel<1>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
+                           e_used, out, ldp, sp, kshift);
     else
-        hipLaunchKernelGGL(split_planes_kernel<3>, g, dim3(256), 0, s, X, ld, mode, rows, K, e, out,
-                           ldp, sp, kshift);
+        hipLaunchKernelGGL(split_planes_kernel<3>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
+                           e_used, out, ldp, sp, kshift);
     return hipGetLastError();
 }
 

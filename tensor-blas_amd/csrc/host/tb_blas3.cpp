@@ -226,13 +226,16 @@ size_t planes_bytes(int npl, long rows, long K, int kshift = 0)
 }
 
 int make_planes(tb_handle_s* h, int npl, const float* X, long ld, int mode, long rows, long K,
-                const int* e, char* mem, Planes* out, int kshift = 0)
+                const int* e, char* mem, Planes* out, int kshift = 0, int* e_used = nullptr)
 {
     out->p = reinterpret_cast<_Float16*>(mem);
     out->ldp = round_up(std::max(1l, K + kshift), 64);
     out->sp = (long long)rows * out->ldp;
-    return hip_status(split_planes(npl, X, ld, mode, (int)rows, (int)K, e, out->p, out->ldp,
-                                   out->sp, kshift, h->stream));
+    hipEvent_t e0;
+    TB_TRY(prof_begin(h, &e0));
+    TB_HIP(split_planes(npl, X, ld, mode, (int)rows, (int)K, e, out->p, out->ldp, out->sp,
+                        kshift, h->stream, e_used));
+    return prof_end(h, kProfPlanes, e0, (double)rows * (4.0 * K + 2.0 * npl * out->ldp));
 }
 
 // One FP16-tier product  C = alpha * op(A) op(B) [+ alpha * op(B-rows) op(A-rows)^T]
@@ -447,6 +450,8 @@ struct TrsmCtx {
     int* eBX;        // running per-row exponent bound of B / X (split leaves and updates)
     int* eL;         // exponent of each row of L (lower part)
     int* eZ;         // exponents of the rows of each W_jj (columns of Z)
+    int* eX;         // the eBX values the planes in xmem were scaled with
+    char* xmem;      // FP16 planes of the solved block read by the current update / leaf
     double* dinv;    // 1 / L(j, j) for every column j
     std::vector<Leaf> leaves;
     std::vector<PlanOp> ops;    // the schedule, recorded by the collecting pass
@@ -459,6 +464,22 @@ struct TrsmCtx {
     char* Bcol(long j) const { return B + j * ldb * es; }
     bool split_exps() const { return upd_tier == kSplitF16 || acc_tier == kSplitF16; }
     bool l_planes() const { return fp16_tier(upd_tier); }
+
+    // op(A) of p := FP16 planes of B(:, k0:k0+kw), scaled with the current eBX, which
+    // the product then reads through its snapshot eX (its own epilogue raises eBX)
+    int x_planes(int tier, long k0, long kw, GemmProb* p)
+    {
+        const bool split = tier == kSplitF16;
+        Planes xp;
+        TB_TRY(make_planes(h, tier_planes(tier), (const float*)Bcol(k0), ldb, kOpN, m, kw,
+                           split ? eBX : nullptr, xmem, &xp, 0, split ? eX : nullptr));
+        p->A = xp.p;
+        p->lda = xp.ldp;
+        p->spA = xp.sp;
+        p->srcA = kSrcPlanes;
+        if (split) p->eA = eX;
+        return TB_SUCCESS;
+    }
 
     // B(:, c0:c0+w) -= B(:, k0:k0+kw) * A(c0:c0+w, k0:k0+kw)^T
     int update(long c0, long w, long k0, long kw, int tier)
@@ -474,6 +495,10 @@ struct TrsmCtx {
                                    (double*)Bcol(c0), ldb, false);
         GemmProb p = make_prob(m, w, kw, Bcol(k0), ldb, kOpN, Aat(c0, k0), lda, kOpT, Bcol(c0), ldb,
                                -1.0, 1.0, kCFull);
+        if (tier == kSplitF16) {
+            p.eA = eBX;
+            p.eB = eL + c0;
+        }
         // the L planes serve updates of the update tier whose column offset is 16-byte
         // aligned and whose width is whole 32-column stages (the kernel copies planes
         // by stages and the planes continue past the block)
@@ -482,10 +507,9 @@ struct TrsmCtx {
             p.ldb = Lp.ldp;
             p.spB = Lp.sp;
             p.srcB = kSrcPlanes;
-        }
-        if (tier == kSplitF16) {
-            p.eA = eBX;
-            p.eB = eL + c0;
+            // the solved block X1 is final: convert it once here rather than in every
+            // tile column of the product
+            TB_TRY(x_planes(tier, k0, kw, &p));
         }
         if (split_exps() && !f64) p.eOut = eBX;
         GemmQueue q(h, tier, kShapeSquare, p.srcA, p.srcB);
@@ -513,9 +537,9 @@ struct TrsmCtx {
             p.ldb = Zp.ldp;
             p.spB = Zp.sp;
             p.srcB = kSrcPlanes;
-            p.eA = eBX;
             p.eB = eZ + eoff[q];
             p.eOut = eBX;
+            TB_TRY(x_planes(acc_tier, off, s, &p));
         }
         GemmQueue g(h, acc_tier, kShapeLeaf, p.srcA, p.srcB);
         TB_TRY(g.add(p));
@@ -584,7 +608,7 @@ int trsm_check(tb_handle_s* h, int tier, long m, long n, const void* A, long lda
 }
 
 struct TrsmLayout {
-    size_t oZ, oE, oD, oZp, oLp, total;
+    size_t oZ, oE, oD, oZp, oLp, oXp, total;
     size_t zsum, esum;
 };
 
@@ -614,10 +638,15 @@ TrsmLayout trsm_plan(TrsmCtx& c)
     t.esum = e;
     Arena a;
     t.oZ = a.reserve(z * c.es);
-    t.oE = a.reserve(sizeof(int) * (c.m + c.n + e));
+    t.oE = a.reserve(sizeof(int) * (2 * c.m + c.n + e));
     t.oD = a.reserve(sizeof(double) * c.n);
     t.oZp = c.acc_tier == kSplitF16 ? a.reserve(planes_bytes(3, (long)e, kLeafMax)) : 0;
     t.oLp = c.l_planes() ? a.reserve(planes_bytes(tier_planes(c.upd_tier), c.n, c.n, c.kshift)) : 0;
+    // planes of the widest solved block an update or a leaf reads
+    long xk = 0;
+    for (const PlanOp& o : c.ops) xk = std::max(xk, o.type == 0 ? o.b : o.d);
+    const int xnpl = std::max(tier_planes(c.upd_tier), tier_planes(c.acc_tier));
+    t.oXp = xnpl > 0 ? a.reserve(planes_bytes(xnpl, c.m, xk)) : 0;
     t.total = a.total() + 1024;
     return t;
 }
@@ -628,6 +657,8 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
     c.eBX = reinterpret_cast<int*>(ws + t.oE);
     c.eL = c.eBX + c.m;
     c.eZ = c.eL + c.n;
+    c.eX = c.eZ + t.esum;
+    c.xmem = ws + t.oXp;
     c.dinv = reinterpret_cast<double*>(ws + t.oD);
     hipStream_t st = c.h->stream;
     const bool acc_f64 = tier_is_f64(c.acc_tier);

@@ -37,7 +37,8 @@ enum ProfCat : int {
     kProfOzakiGemm = 10,
     kProfOzakiSlice = 11,
     kProfTrtri = 12,
-    kProfCats = 13,
+    kProfPlanes = 13,          // FP32 -> FP16 plane conversion (work = bytes moved)
+    kProfCats = 14,
 };
 // Wraps one launch in events when the handle profiles.
 int prof_begin(tb_handle_s* h, hipEvent_t* e0);

@@ -29,10 +29,11 @@ hipError_t gemm_launch(int tier, int shape, int srcA, int srcB, const GemmLaunch
 // out[p*sp + r*ldp + kshift + k].  npl = 1: RNE fp16(x).  npl = 3: with
 // xs = x * 2^(15 - e[r]) (e: row exponents, kExpZero -> 0), xs = h + 2^-11 m + 2^-22 l.
 // ldp must be a multiple of 64 with ldp >= kshift + K; entries of a row outside
-// [kshift, kshift + K) are written as zero.
+// [kshift, kshift + K) are written as zero.  e_used (optional, split tier): receives
+// e[r] as read, the exponents a GEMM must use with these planes when e keeps changing.
 hipError_t split_planes(int npl, const float* X, long long ld, int mode, int rows, int K,
                         const int* e, _Float16* out, long long ldp, long long sp, int kshift,
-                        hipStream_t s);
+                        hipStream_t s, int* e_used = nullptr);
 
 // -------------------------------------------------------------- row exponents
 // Sentinel exponent of an all-zero row.

@@ -108,7 +108,7 @@ class Handle:
 
     PROF_NAMES = ["gemm_plain_sq", "gemm_plain_leaf", "gemm_split_sq", "gemm_split_leaf",
                   "gemm_ozaki_sq", "gemm_ozaki_leaf", "gemm_fp32_sq", "gemm_fp32_leaf",
-                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri"]
+                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri", "planes"]
 
     def set_profiling(self, on):
         _check("tb_set_profiling", lib().tb_set_profiling(self._h, int(on)))
