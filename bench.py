@@ -1,15 +1,24 @@
-"""Headline benchmark: recursive TRSM X*L^T = B at n = m = 16384, nb = 256.
+"""Benchmarks of the MI355X Tensor-BLAS on one GPU; one JSON line on stdout.
 
-The FP32-accurate split tier (hi/lo FP16 MFMA) is timed against the plain FP32
-vector-FMA kernel of the same recursion on the same GPU.  The line also carries the
-plain FP16 tier, the max-relative and normwise error of each tier against an FP64
-host solve of a sample of rows, the dominant kernel's roofline, and host OpenBLAS
-strsm on a bounded sample of the same problem.
+Default workload (the headline): recursive TRSM X*L^T = B at n = m = 16384,
+nb = 256.  The FP32-accurate split tier (3-way FP16 split on MFMA) is timed against
+the plain FP32 vector-FMA kernel of the same recursion on the same GPU.  The line
+also carries the plain FP16 tier, the max-relative and normwise error of each tier
+against an FP64 host solve of a sample of rows, the dominant kernel's roofline and
+host CPU baselines on a bounded sample of the same problem.
 
-One step = restore B from a resident copy (device copy, 2 GiB of HBM traffic) +
+One step = restore B from a resident copy (a device copy, 2 GiB of HBM traffic) +
 one tb_trsm call.  Inputs are resident in HBM before the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (--workload):
+  syrk   plain SYRK C = A A^T, lower, n = k = 2048, nb = 256 (split and FP32-FMA
+         tiers beside it), error vs FP64 dsyrk, host OpenBLAS ssyrk baseline
+  ozaki  FP64 SYR2K by Ozaki INT8 slicing, n = 32768, k = 8192, against the native
+         FP64-MFMA tier; error vs an extended-precision host product on sampled columns
+  gemm   tb_gemm microbenchmark (--m --n --k --tiers)
+  copy   a 256 MiB device copy (reference for HBM byte counters)
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload trsm]
 """
 import argparse
 import json
@@ -25,28 +34,73 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TF = 2516.6      # dense FP16 MFMA, MI355X
+INT8_PEAK_TOPS = 5033.2    # dense INT8 MFMA
+FP64_PEAK_TF = 78.6        # FP64 MFMA
 FP32_VEC_PEAK_TF = 157.3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+# FP16 MFMA products the split tier executes per algorithmic multiply-add
+SPLIT_PRODUCTS = 6
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_inputs(n, m, seed_l=0xC0F2, seed_b=0xC0F3):
-    gl = np.random.default_rng(seed_l)
-    L = gl.uniform(-1.0, 1.0, (n, n)).astype(np.float32)
-    L[np.diag_indices(n)] = (n + gl.uniform(0.0, 1.0, n)).astype(np.float32)
-    gb = np.random.default_rng(seed_b)
-    B = gb.uniform(-1.0, 1.0, (m, n)).astype(np.float32)
-    return L, B     # L's strict upper part is garbage: never read (lower storage)
+def host_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max([i.get("num_threads", 1) for i in threadpool_info()
+                    if i.get("internal_api") == "openblas"] or [os.cpu_count() or 1])
+    except Exception:
+        return os.cpu_count() or 1
 
 
-def host_reference_rows(L, B, rows):
-    """FP64 solve of x*L^T = b for the sampled rows (OpenBLAS dtrsm)."""
-    import scipy.linalg.blas as sb
-    Lf = np.asfortranarray(np.tril(L).astype(np.float64))
-    Bs = np.asfortranarray(B[rows].astype(np.float64))
-    return sb.dtrsm(1.0, Lf, Bs, side=1, lower=1, trans_a=1)
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` in `workload` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel scaled), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                p = line.split()
+                if len(p) >= 4 and not line.startswith("#") and p[0] == kernel and p[1] == workload:
+                    return int(float(p[2]) + float(p[3]))
+    except OSError:
+        pass
+    return None
+
+
+def roofline(prof, peak, unit, bound, workload, per_flop_note):
+    """Roofline of the profiled category with the largest summed duration."""
+    name, (launches, ms, work) = max(prof.items(), key=lambda kv: kv[1][1])
+    achieved = work / (ms * 1e-3) / 1e12
+    roof = {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "traffic": pmc_traffic(name, workload)}
+    info = {"name": name, "launches": launches, "avg_ms": round(ms / launches, 4),
+            "note": "achieved = algorithmic flops of the kernel's launches / their summed "
+                    "duration (HIP events on the library stream); " + per_flop_note}
+    return roof, info
+
+
+def timed(h, fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    h.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    h.synchronize()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def profiled(h, fn):
+    h.set_profiling(True)
+    fn()
+    prof = h.profile()
+    h.set_profiling(False)
+    return prof
 
 
 def err_metrics(X, R):
@@ -55,15 +109,34 @@ def err_metrics(X, R):
             "normwise": float(np.linalg.norm(d) / np.linalg.norm(R))}
 
 
-def cpu_baseline(L, B, budget_s=20.0):
+def line_base(metric, value, unit, steps, warmup, ms, dtype, data, config, hib=True):
+    return {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": hib, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": data, "config": config}
+
+
+# ----------------------------------------------------------------------- TRSM
+def trsm_inputs(n, m, seed_l=0xC0F2, seed_b=0xC0F3):
+    gl = np.random.default_rng(seed_l)
+    L = gl.uniform(-1.0, 1.0, (n, n)).astype(np.float32)
+    L[np.diag_indices(n)] = (n + gl.uniform(0.0, 1.0, n)).astype(np.float32)
+    gb = np.random.default_rng(seed_b)
+    B = gb.uniform(-1.0, 1.0, (m, n)).astype(np.float32)
+    return L, B     # L's strict upper part is never read (lower storage)
+
+
+def trsm_host_rows(L, B, rows):
+    """FP64 solve of x*L^T = b for the sampled rows (OpenBLAS dtrsm)."""
+    import scipy.linalg.blas as sb
+    Lf = np.asfortranarray(np.tril(L).astype(np.float64))
+    Bs = np.asfortranarray(B[rows].astype(np.float64))
+    return sb.dtrsm(1.0, Lf, Bs, side=1, lower=1, trans_a=1)
+
+
+def trsm_cpu_baseline(L, B, budget_s=20.0):
     """Host OpenBLAS strsm (scipy) on the first rows of the same problem."""
     import scipy.linalg.blas as sb
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()
-                       if i.get("internal_api") == "openblas"] or [os.cpu_count() or 1])
-    except Exception:
-        threads = os.cpu_count() or 1
     n = L.shape[0]
     Lf = np.asfortranarray(L)
     rows = 256
@@ -77,30 +150,26 @@ def cpu_baseline(L, B, budget_s=20.0):
         if t_total < 2.0:
             rows *= 2
     tf = done_rows * n * n / t_total / 1e12
-    return {"value": tf, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-            "impl": "OpenBLAS strsm via scipy.linalg.blas (right/lower/T/non-unit, FP32)",
+    return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": int(host_threads()), "kind": "port",
+            "impl": "host OpenBLAS strsm via scipy.linalg.blas (right/lower/T/non-unit, FP32)",
             "sample": f"{done_rows} of {B.shape[0]} rows of B, n={n} ({t_total:.1f} s)"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--m", type=int, default=16384)
-    ap.add_argument("--nb", type=int, default=256)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        sys.exit("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1")
+def trsm_oracle_timing(L, B, rows=8):
+    """The oracle's scalar FP64 restatement of the solve, 1 core, on a few rows."""
+    import oracle
+    t0 = time.perf_counter()
+    oracle.dtrsm_rltn(L, B[:rows])
+    t = time.perf_counter() - t0
+    n = L.shape[0]
+    return {"value": round(rows * n * n / t / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+            "kind": "port", "sample": f"{rows} rows of B, n={n} ({t:.1f} s)"}
 
-    import torch
-    import tensorblas
 
-    n, m, nb = args.n, args.m, args.nb
-    log(f"bench: inputs n={n} m={m} nb={nb}")
-    L, B = make_inputs(n, m)
+def run_trsm(args, torch, tensorblas):
+    n, m, nb = args.n or 16384, args.m or 16384, args.nb
+    log(f"bench: trsm inputs n={n} m={m} nb={nb}")
+    L, B = trsm_inputs(n, m)
     dL = torch.from_numpy(np.ascontiguousarray(L.T)).cuda()     # column-major L, lda = n
     dB0 = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()    # column-major B, ldb = m
     dB = torch.empty_like(dB0)
@@ -108,75 +177,249 @@ def main():
     flops = float(m) * n * n
 
     def step(tier):
-        dB.copy_(dB0)
-        h.trsm(tier, m, n, dL.data_ptr(), n, dB.data_ptr(), m, nb=nb)
-
-    def timed(tier, steps, warmup):
-        for _ in range(warmup):
-            step(tier)
-        h.synchronize()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            step(tier)
-        h.synchronize()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps * 1e3
+        def f():
+            dB.copy_(dB0)
+            h.trsm(tier, m, n, dL.data_ptr(), n, dB.data_ptr(), m, nb=nb)
+        return f
 
     log("bench: split tier (timed)")
-    ms_split = timed("split", args.steps, args.warmup)
+    ms_split = timed(h, step("split"), args.steps, args.warmup)
     rows = np.random.default_rng(7).choice(m, 48, replace=False)
     Xs = dB.cpu().numpy().T[rows]
-
     log("bench: split tier (profiled step)")
-    h.set_profiling(True)
-    step("split")
-    prof = h.profile()
-    h.set_profiling(False)
-
+    prof = profiled(h, step("split"))
     log("bench: fp32 FMA tier")
-    ms_fp32 = timed("fp32", 2, 1)
+    ms_fp32 = timed(h, step("fp32"), 2, 1)
     Xf = dB.cpu().numpy().T[rows]
     log("bench: plain FP16 tier")
-    ms_plain = timed("plain", 3, 1)
+    ms_plain = timed(h, step("plain"), 3, 1)
     Xp = dB.cpu().numpy().T[rows]
-
     log("bench: FP64 host reference rows")
-    R = host_reference_rows(L, B, rows)
+    R = trsm_host_rows(L, B, rows)
     errs = {"split": err_metrics(Xs, R), "fp32": err_metrics(Xf, R), "plain": err_metrics(Xp, R)}
 
-    # dominant kernel: the split update GEMM (128x128 tiles)
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
-    dname, (dlaunch, dms, dflops) = dom
-    achieved = dflops / (dms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_PEAK_TF, "unit": "TFLOP/s",
-            "frac": round(achieved / FP16_PEAK_TF, 4), "traffic": None}
-
+    gemm_prof = {k: v for k, v in prof.items() if k.startswith("gemm_")}
+    roof, rk = roofline(gemm_prof, FP16_PEAK_TF, "TFLOP/s", "mfma", f"trsm{n}",
+                        f"the split tier executes {SPLIT_PRODUCTS} FP16 MFMA products per "
+                        f"algorithmic multiply-add, so its MFMA ceiling in algorithmic flops is "
+                        f"{FP16_PEAK_TF / SPLIT_PRODUCTS:.1f} TFLOP/s")
+    rk["mfma_executed_frac"] = round(roof["frac"] * SPLIT_PRODUCTS, 4)
+    if roof["traffic"]:
+        rk["traffic_source"] = ("profiles/hbm_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
+                                "WRITE_SIZE, scaled by a 256 MiB copy")
     value = flops / (ms_split * 1e-3) / 1e12
-    line = {
-        "metric": "eff. TFLOP/s at FP32-level error vs plain, same GPU; served p50/p99 latency",
-        "value": round(value, 2), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_split, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (fp16x2 split MFMA)",
-        "data": "synthetic: L strictly-lower U[-1,1], diag n+U[0,1]; B U[-1,1] (seeded)",
-        "config": {"workload": "recursive TRSM X*L^T=B, right/lower/T/non-unit, FP32-accurate "
-                               "split tier", "n": n, "m": m, "nb": nb},
+    line = line_base("eff. TFLOP/s at FP32-level error vs plain, same GPU; served p50/p99 latency",
+                     value, "TFLOP/s", args.steps, args.warmup, ms_split,
+                     "f32 (3-way fp16 split MFMA)",
+                     "synthetic: L strictly-lower U[-1,1], diag n+U[0,1]; B U[-1,1] (seeded)",
+                     {"workload": "recursive TRSM X*L^T=B, right/lower/T/non-unit, FP32-accurate "
+                                  "split tier", "n": n, "m": m, "nb": nb})
+    line.update({
         "vs_plain_fp32_fma": {"fp32_fma_tflops": round(flops / (ms_fp32 * 1e-3) / 1e12, 2),
                               "ratio": round(ms_fp32 / ms_split, 3)},
         "plain_fp16_tflops": round(flops / (ms_plain * 1e-3) / 1e12, 2),
         "errors_vs_fp64": errs,
         "roofline": roof,
-        "roofline_kernel": {"name": dname, "launches": dlaunch, "avg_ms": round(dms / dlaunch, 4),
-                            "note": "achieved = algorithmic flops of the kernel's launches / their "
-                                    "summed duration (HIP events on the library stream); the split "
-                                    "tier executes 3 FP16 MFMA products per algorithmic flop"},
+        "roofline_kernel": rk,
         "profile_ms": {k: round(v[1], 3) for k, v in prof.items()},
-    }
+    })
     if not args.no_cpu:
         log("bench: host OpenBLAS strsm sample")
-        line["cpu_baseline"] = cpu_baseline(L, B)
+        line["cpu_baseline"] = trsm_cpu_baseline(L, B)
+        log("bench: oracle restatement sample")
+        line["cpu_oracle"] = trsm_oracle_timing(L, B)
     h.close()
-    print(json.dumps(line), flush=True)
+    return line
+
+
+# ----------------------------------------------------------------------- SYRK
+def run_syrk(args, torch, tensorblas):
+    n, k, nb = args.n or 2048, args.k or (args.n or 2048), args.nb
+    A = np.random.default_rng(0xC0F1).uniform(-1.0, 1.0, (n, k)).astype(np.float32)
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()     # column-major n x k
+    dC = torch.zeros(n * n, device="cuda", dtype=torch.float32)
+    h = tensorblas.Handle()
+    flops = float(n) * n * k
+
+    def step(tier):
+        return lambda: h.syrk(tier, n, k, 1.0, dA.data_ptr(), n, 0.0, dC.data_ptr(), n, nb=nb)
+
+    import scipy.linalg.blas as sb
+    R = sb.dsyrk(1.0, np.asfortranarray(A.astype(np.float64)), lower=1)
+    low = np.tril_indices(n)
+    out, errs = {}, {}
+    for tier in ("plain", "split", "fp32"):
+        out[tier] = timed(h, step(tier), args.steps if tier == "plain" else 3, args.warmup)
+        Cg = dC.cpu().numpy().reshape(n, n).T
+        errs[tier] = err_metrics(Cg[low], R[low])
+    prof = profiled(h, step("plain"))
+    roof, rk = roofline({k_: v for k_, v in prof.items() if k_.startswith("gemm_")}, FP16_PEAK_TF,
+                        "TFLOP/s", "mfma", f"syrk{n}", "plain FP16 tier, one product per flop")
+    ms = out["plain"]
+    line = line_base("eff. TFLOP/s, plain FP16-MFMA SYRK", flops / (ms * 1e-3) / 1e12, "TFLOP/s",
+                     args.steps, args.warmup, ms, "f16 in, f32 accumulate",
+                     "synthetic: A U[-1,1] FP32 (seeded)",
+                     {"workload": "SYRK C=A*A^T lower, alpha=1, beta=0", "n": n, "k": k, "nb": nb})
+    line.update({
+        "tiers_tflops": {t: round(flops / (v * 1e-3) / 1e12, 2) for t, v in out.items()},
+        "split_vs_fp32_fma": round(out["fp32"] / out["split"], 3),
+        "errors_vs_fp64": errs, "roofline": roof, "roofline_kernel": rk,
+        "profile_ms": {k_: round(v[1], 4) for k_, v in prof.items()},
+    })
+    if not args.no_cpu:
+        Af = np.asfortranarray(A)
+        sb.ssyrk(1.0, Af, lower=1)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sb.ssyrk(1.0, Af, lower=1)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        line["cpu_baseline"] = {"value": round(flops / t / 1e12, 4), "unit": "TFLOP/s",
+                                "cores": int(host_threads()), "kind": "port",
+                                "impl": "host OpenBLAS ssyrk via scipy.linalg.blas",
+                                "sample": f"full problem, median of 3 ({t * 1e3:.1f} ms)"}
+    h.close()
+    return line
+
+
+# ---------------------------------------------------------------------- Ozaki
+def ozaki_inputs(n, k, phi, seed):
+    g = np.random.default_rng(seed)
+
+    def mat():
+        return (g.uniform(-0.5, 0.5, (n, k)) * np.exp(phi * g.standard_normal((n, k))))
+    return mat(), mat()
+
+
+def run_ozaki(args, torch, tensorblas):
+    n, k = args.n or 32768, args.k or 8192
+    phi = args.phi
+    A, B = ozaki_inputs(n, k, phi, 0xC0F4)
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()     # column-major n x k
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+    dC = torch.zeros(n * n, device="cuda", dtype=torch.float64)
+    h = tensorblas.Handle()
+    flops = 2.0 * n * n * k
+    cols = np.sort(np.random.default_rng(11).choice(n, 4, replace=False))
+    # extended-precision reference columns: C(i, j) = A(i,:).B(j,:) + B(i,:).A(j,:), i >= j
+    Al, Bl = A.astype(np.longdouble), B.astype(np.longdouble)
+    ref = {int(j): (Al[j:] @ Bl[j] + Bl[j:] @ Al[j]).astype(np.float64) for j in cols}
+
+    def step(tier):
+        return lambda: h.syr2k(tier, n, k, 1.0, dA.data_ptr(), n, dB.data_ptr(), n, 0.0,
+                               dC.data_ptr(), n, nb=256)
+
+    def col_err():
+        Cv = dC.view(n, n)      # Cv[j] = column j
+        worst, num, den = 0.0, 0.0, 0.0
+        for j, r in ref.items():
+            c = Cv[j, j:].cpu().numpy()
+            d = np.abs(c - r)
+            worst = max(worst, float(d.max() / np.abs(r).max()))
+            num += float((d ** 2).sum())
+            den += float((r ** 2).sum())
+        return {"maxrel": worst, "normwise": (num / den) ** 0.5}
+
+    res = {}
+    for s in [int(x) for x in args.slices.split(",")]:
+        h.set_ozaki_slices(s)
+        ms = timed(h, step("ozaki"), args.steps, args.warmup)
+        res[f"ozaki_s{s}"] = {"ms": round(ms, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                              "errors_vs_ref": col_err()}
+    ms64 = timed(h, step("fp64"), 2, 1)
+    res["fp64_mfma"] = {"ms": round(ms64, 2), "tflops": round(flops / (ms64 * 1e-3) / 1e12, 2),
+                        "errors_vs_ref": col_err()}
+    best = min((v["ms"], key) for key, v in res.items()
+               if key != "fp64_mfma" and v["errors_vs_ref"]["maxrel"] <= 1e-14)
+    sbest = int(best[1].split("_s")[1])
+    h.set_ozaki_slices(sbest)
+    prof = profiled(h, step("ozaki"))
+    oz = {k_: v for k_, v in prof.items() if k_ == "ozaki_gemm"}
+    roof, rk = roofline(oz, FP64_PEAK_TF, "TFLOP/s", "mfma", f"syr2k{n}x{k}",
+                        f"FP64-equivalent flops; {sbest} slices run {sbest * (sbest + 1) // 2} "
+                        f"INT8 MFMA slice products per flop (INT8 peak {INT8_PEAK_TOPS} TOPS)")
+    ms = best[0]
+    line = line_base("eff. FP64 TFLOP/s of Ozaki SYR2K at <=1e-14 max relative error",
+                     flops / (ms * 1e-3) / 1e12, "TFLOP/s", args.steps, args.warmup, ms,
+                     "f64 (int8 slices, fp64 accumulate)",
+                     f"synthetic: U[-0.5,0.5]*exp({phi}*N(0,1)) FP64 (seeded)",
+                     {"workload": "SYR2K C=A*B^T+B*A^T lower, Ozaki INT8 slicing", "n": n, "k": k,
+                      "phi": phi, "slices": sbest})
+    line.update({"results": res, "vs_fp64_mfma": round(ms64 / ms, 3), "roofline": roof,
+                 "roofline_kernel": rk, "profile_ms": {k_: round(v[1], 3) for k_, v in prof.items()},
+                 "reference": "numpy longdouble products on 4 sampled columns (lower part)"})
+    h.close()
+    return line
+
+
+# ----------------------------------------------------------------- microbench
+def run_gemm(args, torch, tensorblas):
+    M, N, K = args.m or 8192, args.n or 8192, args.k or 8192
+    h = tensorblas.Handle()
+    res = {}
+    for tier in args.tiers.split(","):
+        dt = torch.float64 if tier in ("fp64", "ozaki") else torch.float32
+        A = torch.rand(K, M, device="cuda", dtype=dt) * 2 - 1   # column-major M x K
+        B = torch.rand(N, K, device="cuda", dtype=dt) * 2 - 1   # column-major K x N
+        C = torch.zeros(N, M, device="cuda", dtype=dt)
+
+        def f():
+            h.gemm(tier, 0, 0, M, N, K, 1.0, A.data_ptr(), M, B.data_ptr(), K, 0.0, C.data_ptr(), M)
+        ms = timed(h, f, args.steps, args.warmup)
+        prof = profiled(h, f)
+        res[tier] = {"ms": round(ms, 3), "tflops": round(2.0 * M * N * K / (ms * 1e-3) / 1e12, 2),
+                     "kernels": {k_: {"avg_ms": round(v[1] / v[0], 4),
+                                      "tflops": round(v[2] / (v[1] * 1e-3) / 1e12, 2)}
+                                 for k_, v in prof.items()}}
+        log(f"bench: gemm {tier} {M}x{N}x{K}: {res[tier]}")
+        del A, B, C
+    first = res[args.tiers.split(",")[0]]
+    line = line_base("GEMM TFLOP/s (microbenchmark)", first["tflops"], "TFLOP/s", args.steps,
+                     args.warmup, first["ms"], "per tier", "synthetic U[-1,1]",
+                     {"workload": "tb_gemm NN", "m": M, "n": N, "k": K})
+    line["tiers"] = res
+    h.close()
+    return line
+
+
+def run_copy(args, torch, tensorblas):
+    x = torch.rand(64 * 1024 * 1024, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / 3 * 1e3
+    gbs = 2 * x.numel() * 4 / (ms * 1e-3) / 1e9
+    return line_base("device copy GB/s", gbs, "GB/s", 3, 0, ms, "f32", "synthetic",
+                     {"workload": "256 MiB device-to-device copy"})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="trsm", choices=["trsm", "syrk", "ozaki", "gemm", "copy"])
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--m", type=int, default=0)
+    ap.add_argument("--k", type=int, default=0)
+    ap.add_argument("--nb", type=int, default=256)
+    ap.add_argument("--tiers", default="plain,split,fp32")
+    ap.add_argument("--slices", default="6,7,8,9")
+    ap.add_argument("--phi", type=float, default=0.5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1")
+
+    import torch
+    import tensorblas
+    run = {"trsm": run_trsm, "syrk": run_syrk, "ozaki": run_ozaki, "gemm": run_gemm,
+           "copy": run_copy}[args.workload]
+    print(json.dumps(run(args, torch, tensorblas)), flush=True)
 
 
 if __name__ == "__main__":

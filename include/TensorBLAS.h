@@ -49,8 +49,9 @@ typedef struct tb_handle_s* tb_handle_t;
 /* Precision tiers.  TB_TIER_PLAIN is the reference's tc_* family (FP32 operands
  * converted to FP16 round-to-nearest-even, FP16 MFMA, FP32 accumulation);
  * TB_TIER_SPLIT is the tc_cumpsgemm_* family (FP32-accurate: each FP32 operand is
- * scaled per row by a power of two and split into FP16 hi + 2^-11 lo, three MFMA
- * products); TB_TIER_OZAKI is the tc_ozimmu_* family (FP64 operands sliced into
+ * scaled per row by a power of two and split exactly into FP16 hi + 2^-11 mid +
+ * 2^-22 lo; the six products of weight >= 2^-22 run on FP16 MFMA into three FP32
+ * accumulators); TB_TIER_OZAKI is the tc_ozimmu_* family (FP64 operands sliced into
  * INT8 digits, INT8 MFMA with exact int32 sums, FP64 accumulation).  The two
  * comparators have no reference family: TB_TIER_FP32 runs the same recursion on
  * FP32 vector FMA, TB_TIER_FP64 on native FP64 MFMA. */
