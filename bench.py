@@ -1,7 +1,7 @@
 """Benchmarks of the MI355X Tensor-BLAS on one GPU; one JSON line on stdout.
 
 Default workload (the headline): recursive TRSM X*L^T = B at n = m = 16384,
-nb = 256.  The FP32-accurate split tier (3-way FP16 split on MFMA) is timed against
+nb = 256.  The FP32-accurate split tier (FP16 hi/lo split on MFMA) is timed against
 the plain FP32 vector-FMA kernel of the same recursion on the same GPU.  The line
 also carries the plain FP16 tier, the max-relative and normwise error of each tier
 against an FP64 host solve of a sample of rows, the dominant kernel's roofline and
@@ -38,8 +38,8 @@ INT8_PEAK_TOPS = 5033.2    # dense INT8 MFMA
 FP64_PEAK_TF = 78.6        # FP64 MFMA
 FP32_VEC_PEAK_TF = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
-# FP16 MFMA products the split tier executes per algorithmic multiply-add
-SPLIT_PRODUCTS = 6
+# FP16 MFMA products the split tier executes per algorithmic multiply-add, by planes
+SPLIT_PRODUCTS = {2: 3, 3: 6}
 
 
 def log(msg):
@@ -146,7 +146,7 @@ def trsm_cpu_baseline(L, B, budget_s=20.0):
         t0 = time.perf_counter()
         sb.strsm(1.0, Lf, Bs, side=1, lower=1, trans_a=1, overwrite_b=1)
         t_total += time.perf_counter() - t0
-        done_rows += rows
+        done_rows += Bs.shape[0]
         if t_total < 2.0:
             rows *= 2
     tf = done_rows * n * n / t_total / 1e12
@@ -155,7 +155,7 @@ def trsm_cpu_baseline(L, B, budget_s=20.0):
             "sample": f"{done_rows} of {B.shape[0]} rows of B, n={n} ({t_total:.1f} s)"}
 
 
-def trsm_oracle_timing(L, B, rows=8):
+def trsm_oracle_timing(L, B, rows=4):
     """The oracle's scalar FP64 restatement of the solve, 1 core, on a few rows."""
     import oracle
     t0 = time.perf_counter()
@@ -174,6 +174,7 @@ def run_trsm(args, torch, tensorblas):
     dB0 = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()    # column-major B, ldb = m
     dB = torch.empty_like(dB0)
     h = tensorblas.Handle()
+    h.set_split_planes(args.split_planes)
     flops = float(m) * n * n
 
     def step(tier):
@@ -199,21 +200,23 @@ def run_trsm(args, torch, tensorblas):
     errs = {"split": err_metrics(Xs, R), "fp32": err_metrics(Xf, R), "plain": err_metrics(Xp, R)}
 
     gemm_prof = {k: v for k, v in prof.items() if k.startswith("gemm_")}
+    products = SPLIT_PRODUCTS[args.split_planes]
     roof, rk = roofline(gemm_prof, FP16_PEAK_TF, "TFLOP/s", "mfma", f"trsm{n}",
-                        f"the split tier executes {SPLIT_PRODUCTS} FP16 MFMA products per "
-                        f"algorithmic multiply-add, so its MFMA ceiling in algorithmic flops is "
-                        f"{FP16_PEAK_TF / SPLIT_PRODUCTS:.1f} TFLOP/s")
-    rk["mfma_executed_frac"] = round(roof["frac"] * SPLIT_PRODUCTS, 4)
+                        f"the split tier ({args.split_planes} FP16 planes) executes {products} "
+                        f"FP16 MFMA products per algorithmic multiply-add, so its MFMA ceiling in "
+                        f"algorithmic flops is {FP16_PEAK_TF / products:.1f} TFLOP/s")
+    rk["mfma_executed_frac"] = round(roof["frac"] * products, 4)
     if roof["traffic"]:
         rk["traffic_source"] = ("profiles/hbm_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
                                 "WRITE_SIZE, scaled by a 256 MiB copy")
     value = flops / (ms_split * 1e-3) / 1e12
     line = line_base("eff. TFLOP/s at FP32-level error vs plain, same GPU; served p50/p99 latency",
                      value, "TFLOP/s", args.steps, args.warmup, ms_split,
-                     "f32 (3-way fp16 split MFMA)",
+                     f"f32 ({args.split_planes}-plane fp16 split MFMA)",
                      "synthetic: L strictly-lower U[-1,1], diag n+U[0,1]; B U[-1,1] (seeded)",
                      {"workload": "recursive TRSM X*L^T=B, right/lower/T/non-unit, FP32-accurate "
-                                  "split tier", "n": n, "m": m, "nb": nb})
+                                  "split tier", "n": n, "m": m, "nb": nb,
+                      "split_planes": args.split_planes})
     line.update({
         "vs_plain_fp32_fma": {"fp32_fma_tflops": round(flops / (ms_fp32 * 1e-3) / 1e12, 2),
                               "ratio": round(ms_fp32 / ms_split, 3)},
@@ -239,6 +242,7 @@ def run_syrk(args, torch, tensorblas):
     dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()     # column-major n x k
     dC = torch.zeros(n * n, device="cuda", dtype=torch.float32)
     h = tensorblas.Handle()
+    h.set_split_planes(args.split_planes)
     flops = float(n) * n * k
 
     def step(tier):
@@ -357,6 +361,7 @@ def run_ozaki(args, torch, tensorblas):
 def run_gemm(args, torch, tensorblas):
     M, N, K = args.m or 8192, args.n or 8192, args.k or 8192
     h = tensorblas.Handle()
+    h.set_split_planes(args.split_planes)
     res = {}
     for tier in args.tiers.split(","):
         dt = torch.float64 if tier in ("fp64", "ozaki") else torch.float32
@@ -410,6 +415,7 @@ def main():
     ap.add_argument("--tiers", default="plain,split,fp32")
     ap.add_argument("--slices", default="6,7,8,9")
     ap.add_argument("--phi", type=float, default=0.5)
+    ap.add_argument("--split-planes", type=int, default=2, choices=[2, 3])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -49,9 +49,9 @@ typedef struct tb_handle_s* tb_handle_t;
 /* Precision tiers.  TB_TIER_PLAIN is the reference's tc_* family (FP32 operands
  * converted to FP16 round-to-nearest-even, FP16 MFMA, FP32 accumulation);
  * TB_TIER_SPLIT is the tc_cumpsgemm_* family (FP32-accurate: each FP32 operand is
- * scaled per row by a power of two and split exactly into FP16 hi + 2^-11 mid +
- * 2^-22 lo; the six products of weight >= 2^-22 run on FP16 MFMA into three FP32
- * accumulators); TB_TIER_OZAKI is the tc_ozimmu_* family (FP64 operands sliced into
+ * scaled per row by a power of two and split into FP16 hi + 2^-11 lo; the three
+ * products hh | hm+mh run on FP16 MFMA into two FP32 accumulators; see
+ * tb_set_split_planes for the exact three-plane form); TB_TIER_OZAKI is the tc_ozimmu_* family (FP64 operands sliced into
  * INT8 digits, INT8 MFMA with exact int32 sums, FP64 accumulation).  The two
  * comparators have no reference family: TB_TIER_FP32 runs the same recursion on
  * FP32 vector FMA, TB_TIER_FP64 on native FP64 MFMA. */
@@ -78,6 +78,12 @@ TB_API int tb_set_stream(tb_handle_t handle, void* stream);
 TB_API int tb_synchronize(tb_handle_t handle);
 /* Ozaki tier: number of INT8 slices (3..10, default 7) */
 TB_API int tb_set_ozaki_slices(tb_handle_t handle, int nslices);
+/* Split tier: FP16 planes per FP32 operand (default 2).  2: x*2^s = h + 2^-11 m
+ * (23 significant bits for each row's largest entries), three MFMA products
+ * (hh | hm+mh): the published FP16 error-correction scheme.  3: h + 2^-11 m + 2^-22 l,
+ * exact for FP32 inputs, six products (hh | hm+mh | hl+mm+lh) at twice the MFMA work,
+ * for inputs whose rows span many binades. */
+TB_API int tb_set_split_planes(tb_handle_t handle, int nplanes);
 /* bytes of device workspace the handle currently holds */
 TB_API long tb_workspace_bytes(tb_handle_t handle);
 /* Per-launch profiling with HIP events on the handle's stream (off by default;

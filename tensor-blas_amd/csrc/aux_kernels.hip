@@ -229,7 +229,7 @@ split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int row
     const int r = r0 + rl;
     if (r >= rows) return;
     int sc = 0;
-    if (NPL == 3) {
+    if (NPL >= 2) {
         const int ex = e[r];
         sc = 15 - (ex < -1000 ? 0 : (ex > 1000 ? 1000 : ex));
         if (e_used && blockIdx.x == 0 && (tid & 3) == 0) e_used[r] = ex;
@@ -247,18 +247,15 @@ split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int row
                 const _Float16 h = (_Float16)xs;
                 const float r1 = (xs - (float)h) * 2048.0f;
                 const _Float16 m = (_Float16)r1;
-                const float r2 = (r1 - (float)m) * 2048.0f;
                 hh[j] = h;
                 mm[j] = m;
-                ll[j] = (_Float16)r2;
+                if (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
             }
         }
         _Float16* dst = out + (long long)r * ldp + c0 + cl0 + half * 8;
         *reinterpret_cast<half8*>(dst) = hh;
-        if (NPL == 3) {
-            *reinterpret_cast<half8*>(dst + sp) = mm;
-            *reinterpret_cast<half8*>(dst + 2 * sp) = ll;
-        }
+        if (NPL >= 2) *reinterpret_cast<half8*>(dst + sp) = mm;
+        if (NPL == 3) *reinterpret_cast<half8*>(dst + 2 * sp) = ll;
     }
 }
 
@@ -279,12 +276,15 @@ hipError_t split_planes(int npl, const float* X, long long ld, int mode, int row
                         hipStream_t s, int* e_used)
 {
     if (rows <= 0) return hipSuccess;
-    if (ldp % 64 || kshift < 0 || ldp < kshift + K || (npl != 1 && npl != 3) ||
-        (reinterpret_cast<uintptr_t>(out) % 16) || (npl == 3 && sp % 8))
+    if (ldp % 64 || kshift < 0 || ldp < kshift + K || npl < 1 || npl > 3 ||
+        (reinterpret_cast<uintptr_t>(out) % 16) || (npl > 1 && sp % 8))
         return hipErrorInvalidValue;
     dim3 g((unsigned)(ldp / 64), (unsigned)((rows + 63) / 64));
     if (npl == 1)
         hipLaunchKernelGGL(split_planes_kernel<1>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
+                           e_used, out, ldp, sp, kshift);
+    else if (npl == 2)
+        hipLaunchKernelGGL(split_planes_kernel<2>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
                            e_used, out, ldp, sp, kshift);
     else
         hipLaunchKernelGGL(split_planes_kernel<3>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,

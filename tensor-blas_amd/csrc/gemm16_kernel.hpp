@@ -19,6 +19,8 @@
 // and the six products of total weight >= 2^-22 go to three FP32 accumulators:
 //   acc0 += hh;  acc1 += hm + mh;  acc2 += hl + mm + lh;
 // C = (acc0 + 2^-11 (acc1 + 2^-11 acc2)) * 2^-(scA + scB).
+// NPL = 2 (split tier, two planes): xs = h + 2^-11 m + r with |r| <= 2^-22 |xs|max
+// (23 significant bits for the row's largest entries); acc0 += hh, acc1 += hm + mh.
 //
 // LDS: every plane of a panel is [row][32 halves] = 64-byte rows, unpadded; the four
 // 16-byte units of row r sit at unit position u ^ ((r >> 2) & 3).  Both the 16-lane
@@ -221,14 +223,13 @@ struct Panel16 {
                         const _Float16 h = (_Float16)xs;
                         const float r1 = (xs - (float)h) * 2048.0f;      // exact
                         const _Float16 m = (_Float16)r1;
-                        const float r2 = (r1 - (float)m) * 2048.0f;      // exact
                         hh[j] = h;
                         mm[j] = m;
-                        ll[j] = (_Float16)r2;
+                        if constexpr (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
                     }
                     *reinterpret_cast<half8_t*>(dst) = hh;
                     *reinterpret_cast<half8_t*>(dst + PLANE_BYTES) = mm;
-                    *reinterpret_cast<half8_t*>(dst + 2 * PLANE_BYTES) = ll;
+                    if constexpr (NPL == 3) *reinterpret_cast<half8_t*>(dst + 2 * PLANE_BYTES) = ll;
                 }
             }
         }
@@ -246,11 +247,12 @@ gemm16_kernel(GemmLaunch L)
     constexpr int FM = WTM / 32, FN = WTN / 32;
     static_assert(FM >= 1 && FN >= 1, "wave tile");
     constexpr int STAGE = PA::BYTES + PB::BYTES;
-    constexpr bool kSplit = (NPL == 3);
+    constexpr bool kSplit = (NPL >= 2);     // 2 or 3 planes: split tier
+    constexpr bool kLo = (NPL == 3);        // third plane and accumulator
     // direct-to-LDS instructions each wave issues per stage (the vmcnt of one stage)
     constexpr int DMA_W = (SRCA == kSrcPlanes ? PA::DMA_PER_WAVE : 0) +
                           (SRCB == kSrcPlanes ? PB::DMA_PER_WAVE : 0);
-    static_assert(NST >= 2 && NST <= 3, "stages");
+    static_assert(NST >= 2 && NST <= 4, "stages");
     // loads that may stay in flight when stage s+1 must be complete: those of stages
     // s+2 .. s+NST-1 (issued later, completing later: vmcnt retires in order)
     constexpr int KEEP = (NST - 2) * DMA_W;
@@ -376,14 +378,13 @@ gemm16_kernel(GemmLaunch L)
     constexpr int NACC = FM * FN;
     floatx16_t acc0[NACC];
     floatx16_t acc1[kSplit ? NACC : 1];
-    floatx16_t acc2[kSplit ? NACC : 1];
+    floatx16_t acc2[kLo ? NACC : 1];
 #pragma unroll
     for (int q = 0; q < NACC; ++q) acc0[q] = floatx16_t{};
 #pragma unroll
-    for (int q = 0; q < (kSplit ? NACC : 1); ++q) {
-        acc1[q] = floatx16_t{};
-        acc2[q] = floatx16_t{};
-    }
+    for (int q = 0; q < (kSplit ? NACC : 1); ++q) acc1[q] = floatx16_t{};
+#pragma unroll
+    for (int q = 0; q < (kLo ? NACC : 1); ++q) acc2[q] = floatx16_t{};
 
     // fragment addressing: lane reads row (fragment base + lane%32), 16-byte unit
     // kk/8 + lane/32, swizzled by bits 2..3 of the row (the base is a multiple of 32)
@@ -416,7 +417,7 @@ gemm16_kernel(GemmLaunch L)
 #pragma unroll
         for (int q = 0; q < NACC; ++q)
             acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][0], acc0[q], 0, 0, 0);
-        if constexpr (kSplit) {
+        if constexpr (kLo) {
 #pragma unroll
             for (int q = 0; q < NACC; ++q)
                 acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][1], acc1[q], 0, 0, 0);
@@ -432,6 +433,14 @@ gemm16_kernel(GemmLaunch L)
 #pragma unroll
             for (int q = 0; q < NACC; ++q)
                 acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][2], f.a[q % FM][0], acc2[q], 0, 0, 0);
+        } else if constexpr (kSplit) {
+            // two planes: hm + mh into acc1 (the 2^-22 products are not formed)
+#pragma unroll
+            for (int q = 0; q < NACC; ++q)
+                acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][1], acc1[q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < NACC; ++q)
+                acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][1], f.a[q % FM][0], acc1[q], 0, 0, 0);
         }
     };
 
@@ -446,8 +455,12 @@ gemm16_kernel(GemmLaunch L)
         for (int s = 0; s < NST - 1; ++s)
             if (s < nst) issue_dma(s);
         if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32) store_regs(0);
-        // stage 0 complete: only stages 1..NST-2 may still be in flight
-        if (KEEP > 0 && nst > 1) TB_WAIT_VM_BARRIER(KEEP);
+        // stage 0 complete: only the other issued stages (1 .. issued-1) may still be
+        // in flight, DMA_W loads each
+        const int issued = nst < NST - 1 ? nst : NST - 1;
+        if (issued == NST - 1) TB_WAIT_VM_BARRIER(KEEP);
+        else if (issued == 3) TB_WAIT_VM_BARRIER(2 * DMA_W);
+        else if (issued == 2) TB_WAIT_VM_BARRIER(DMA_W);
         else TB_WAIT_VM_BARRIER(0);
 #pragma unroll 1
         for (int s = 0; s < nst; ++s) {
@@ -490,7 +503,8 @@ gemm16_kernel(GemmLaunch L)
                 const int j = j0 + jl;
                 float v = acc0[q][r];
                 if constexpr (kSplit) {
-                    const float t = __builtin_fmaf(acc2[q][r], 1.0f / 2048.0f, acc1[q][r]);
+                    float t = acc1[q][r];
+                    if constexpr (kLo) t = __builtin_fmaf(acc2[q][r], 1.0f / 2048.0f, t);
                     v = __builtin_ldexpf(__builtin_fmaf(t, 1.0f / 2048.0f, v), -(scA + scB_s[jl]));
                 }
                 if (i < M && j < N && (!lowerC || i >= j)) {

@@ -26,10 +26,11 @@ hipError_t run16(const GemmLaunch& L, hipStream_t s)
 template <int NPL, int SA, int SB>
 hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s)
 {
-    // four waves, 64 x 64 accumulator tile per wave; three LDS stages where they fit
-    // in 160 KB (the split leaf tile holds two)
-    return shape == kShapeLeaf ? run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s)
-                               : run16<NPL, SA, SB, 128, 128, 2, 2, 3>(L, s);
+    // four waves, 64 x 64 accumulator tile per wave; a stage is (BM + BN) x 64 B per
+    // plane, and the ring holds as many stages (<= 4) as fit in the 160 KB of LDS
+    if (shape == kShapeLeaf)
+        return run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s);
+    return run16<NPL, SA, SB, 128, 128, 2, 2, NPL == 3 ? 3 : 4>(L, s);
 }
 
 template <int NPL>
@@ -50,14 +51,21 @@ TileShape gemm_tile_shape(int tier, int shape)
     return TileShape{128, 128};
 }
 
-int tier_planes(int tier) { return tier == kSplitF16 ? 3 : (tier == kPlainF16 ? 1 : 0); }
+int tier_planes(int tier, int split_planes)
+{
+    return tier == kSplitF16 ? split_planes : (tier == kPlainF16 ? 1 : 0);
+}
 
-hipError_t gemm_launch(int tier, int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s)
+hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const GemmLaunch& L,
+                       hipStream_t s)
 {
     if (L.totalTiles <= 0) return hipSuccess;
     switch (tier) {
     case kPlainF16: return run16_src<1>(shape, srcA, srcB, L, s);
-    case kSplitF16: return run16_src<3>(shape, srcA, srcB, L, s);
+    case kSplitF16:
+        if (npl == 2) return run16_src<2>(shape, srcA, srcB, L, s);
+        if (npl == 3) return run16_src<3>(shape, srcA, srcB, L, s);
+        return hipErrorInvalidValue;
     case kFmaF32:
         if (srcA != kSrcF32 || srcB != kSrcF32) return hipErrorInvalidValue;
         return shape == kShapeLeaf ? run<kFmaF32, 64, 256, 32, 1, 4>(L, s)

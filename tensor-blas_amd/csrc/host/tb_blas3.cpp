@@ -132,7 +132,7 @@ int GemmQueue::flush()
     for (int q = 0; q < L_.np; ++q) flops += prob_flops(L_.p[q]);
     hipEvent_t e0;
     TB_TRY(prof_begin(h_, &e0));
-    hipError_t e = gemm_launch(tier_, shape_, srcA_, srcB_, L_, h_->stream);
+    hipError_t e = gemm_launch(tier_, planes_of(h_, tier_), shape_, srcA_, srcB_, L_, h_->stream);
     std::memset(&L_, 0, sizeof(L_));
     TB_HIP(e);
     return prof_end(h_, kProfGemm0 + tier_ * 2 + shape_, e0, flops);
@@ -251,7 +251,7 @@ struct RowsOf {
 int fp16_product(tb_handle_s* h, int tier, long m, long n, long k, double alpha, RowsOf a,
                  RowsOf b, bool same_ab, bool pairs, double beta, float* C, long ldc, int modeC)
 {
-    const int npl = tier_planes(tier);
+    const int npl = planes_of(h, tier);
     const bool split = tier == kSplitF16;
     const bool sep_b = !same_ab;
     Arena ar;
@@ -471,7 +471,7 @@ struct TrsmCtx {
     {
         const bool split = tier == kSplitF16;
         Planes xp;
-        TB_TRY(make_planes(h, tier_planes(tier), (const float*)Bcol(k0), ldb, kOpN, m, kw,
+        TB_TRY(make_planes(h, planes_of(h, tier), (const float*)Bcol(k0), ldb, kOpN, m, kw,
                            split ? eBX : nullptr, xmem, &xp, 0, split ? eX : nullptr));
         p->A = xp.p;
         p->lda = xp.ldp;
@@ -640,12 +640,12 @@ TrsmLayout trsm_plan(TrsmCtx& c)
     t.oZ = a.reserve(z * c.es);
     t.oE = a.reserve(sizeof(int) * (2 * c.m + c.n + e));
     t.oD = a.reserve(sizeof(double) * c.n);
-    t.oZp = c.acc_tier == kSplitF16 ? a.reserve(planes_bytes(3, (long)e, kLeafMax)) : 0;
-    t.oLp = c.l_planes() ? a.reserve(planes_bytes(tier_planes(c.upd_tier), c.n, c.n, c.kshift)) : 0;
+    t.oZp = c.acc_tier == kSplitF16 ? a.reserve(planes_bytes(planes_of(c.h, kSplitF16), (long)e, kLeafMax)) : 0;
+    t.oLp = c.l_planes() ? a.reserve(planes_bytes(planes_of(c.h, c.upd_tier), c.n, c.n, c.kshift)) : 0;
     // planes of the widest solved block an update or a leaf reads
     long xk = 0;
     for (const PlanOp& o : c.ops) xk = std::max(xk, o.type == 0 ? o.b : o.d);
-    const int xnpl = std::max(tier_planes(c.upd_tier), tier_planes(c.acc_tier));
+    const int xnpl = std::max(planes_of(c.h, c.upd_tier), planes_of(c.h, c.acc_tier));
     t.oXp = xnpl > 0 ? a.reserve(planes_bytes(xnpl, c.m, xk)) : 0;
     t.total = a.total() + 1024;
     return t;
@@ -668,7 +668,7 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
         TB_HIP(rowexp(false, c.A, c.lda, kOpTriL, (int)c.n, (int)c.n, 1, 0, c.eL, 0, st));
     }
     if (c.l_planes()) {
-        const int npl = tier_planes(c.upd_tier);
+        const int npl = planes_of(c.h, c.upd_tier);
         TB_TRY(make_planes(c.h, npl, (const float*)c.A, c.lda, kOpTriL, c.n, c.n, c.eL,
                            ws + t.oLp, &c.Lp, c.kshift));
     }
@@ -696,7 +696,7 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
             int* ez = c.eZ + c.eoff[q];
             const float* zq = reinterpret_cast<const float*>(c.Z + c.zoff[q] * c.es);
             TB_HIP(rowexp(false, zq, s, kOpT, (int)(s * cnt), (int)s, 1, 0, ez, 0, st));
-            TB_HIP(split_planes(3, zq, s, kOpT, (int)(s * cnt), (int)s, ez,
+            TB_HIP(split_planes(planes_of(c.h, kSplitF16), zq, s, kOpT, (int)(s * cnt), (int)s, ez,
                                 c.Zp.p + (long long)c.eoff[q] * c.Zp.ldp, c.Zp.ldp, c.Zp.sp, 0, st));
         }
         q = r;

@@ -88,6 +88,13 @@ int tb_set_ozaki_slices(tb_handle_t h, int nslices)
     return TB_SUCCESS;
 }
 
+int tb_set_split_planes(tb_handle_t h, int nplanes)
+{
+    if (!h || nplanes < 2 || nplanes > 3) return TB_ERR_INVALID;
+    h->split_planes = nplanes;
+    return TB_SUCCESS;
+}
+
 long tb_workspace_bytes(tb_handle_t h) { return h ? (long)(h->ws_bytes[0] + h->ws_bytes[1]) : 0; }
 
 const char* tb_status_string(int status)

@@ -23,6 +23,7 @@ struct tb_handle_s {
     std::vector<tb_prof_rec> prof_recs;
     int device = 0;
     int ozaki_slices = 7;
+    int split_planes = 2;   // FP16 planes per operand of the split tier (2 or 3)
     // device workspaces, grown on demand, freed by tb_destroy: slot 0 for routines,
     // slot 1 for the Ozaki GEMM (which routines call with their own slot live)
     void* ws[2] = {nullptr, nullptr};
@@ -73,6 +74,12 @@ int workspace_get(tb_handle_s* h, size_t bytes, char** base, int slot = 0);
 // Collects GEMM problems of one tier, tile shape and operand source kinds (all
 // problems must match them); launches up to kMaxProbs problems per kernel launch,
 // in order.
+// FP16 planes an operand of `tier` is converted to on handle h (h may be null: default)
+inline int planes_of(const tb_handle_s* h, int tier)
+{
+    return tier_planes(tier, h ? h->split_planes : 2);
+}
+
 class GemmQueue {
 public:
     GemmQueue(tb_handle_s* h, int tier, int shape, int srcA = 0, int srcB = 0);

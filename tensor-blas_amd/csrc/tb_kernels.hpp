@@ -19,10 +19,13 @@ struct TileShape {
     int bm, bn;
 };
 TileShape gemm_tile_shape(int tier, int shape);
-// FP16 planes per operand element of a tier: plain 1, split 3, others 0
-int tier_planes(int tier);
-// all problems of L share the operand source kinds srcA/srcB (GemmProb::srcA/srcB)
-hipError_t gemm_launch(int tier, int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s);
+// FP16 planes per operand element of a tier: plain 1, split `split_planes` (2 or 3),
+// others 0
+int tier_planes(int tier, int split_planes);
+// all problems of L share the operand source kinds srcA/srcB (GemmProb::srcA/srcB);
+// npl: planes of the split tier (2 or 3)
+hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const GemmLaunch& L,
+                       hipStream_t s);
 
 // Pre-conversion of the rows of op(X) (FP32, rowexp modes kOpN/kOpT/kOpSymL/kOpTriL,
 // rows x K) into npl K-contiguous FP16 planes: element (r, k) of plane p at

@@ -43,6 +43,7 @@ def lib():
         sig = {
             "tb_create": [_P], "tb_destroy": [_P], "tb_set_stream": [_P, _P],
             "tb_synchronize": [_P], "tb_set_ozaki_slices": [_P, _I],
+            "tb_set_split_planes": [_P, _I],
             "tb_syrk": [_P, _I, _L, _L, _D, _P, _L, _D, _P, _L, _L],
             "tb_syr2k": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _L],
             "tb_trmm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
@@ -105,6 +106,9 @@ class Handle:
 
     def set_ozaki_slices(self, s):
         _check("tb_set_ozaki_slices", lib().tb_set_ozaki_slices(self._h, s))
+
+    def set_split_planes(self, n):
+        _check("tb_set_split_planes", lib().tb_set_split_planes(self._h, n))
 
     PROF_NAMES = ["gemm_plain_sq", "gemm_plain_leaf", "gemm_split_sq", "gemm_split_leaf",
                   "gemm_ozaki_sq", "gemm_ozaki_leaf", "gemm_fp32_sq", "gemm_fp32_leaf",

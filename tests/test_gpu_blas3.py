@@ -43,8 +43,19 @@ def absprod(A, B):
     return oracle.dgemm(np.abs(A), np.abs(B))
 
 
-def assert_bound(C, R, S, kfac, u, extra=3.0, lower=False, what=""):
-    bound = (gamma(max(kfac, 1), u) + extra * u) * S
+def split_rep(opA, opB):
+    """Representation error of the default (two-plane) split tier: each row-scaled
+    operand is h + 2^-11 m, off by at most 2^-22 of its row's (column's) largest
+    magnitude per element, so |dC| gains 2^-22 (max_row|A| * sum|B| + sum|A| * max_col|B|)
+    (with a factor 2 margin)."""
+    ra = np.abs(opA).max(axis=1, initial=0.0)[:, None]
+    cb = np.abs(opB).max(axis=0, initial=0.0)[None, :]
+    return 2.0 ** -21 * (ra * np.abs(opB).sum(axis=0)[None, :] +
+                         np.abs(opA).sum(axis=1)[:, None] * cb)
+
+
+def assert_bound(C, R, S, kfac, u, extra=3.0, lower=False, what="", extra_abs=0.0):
+    bound = (gamma(max(kfac, 1), u) + extra * u) * S + extra_abs
     err = np.abs(C.astype(np.float64) - R)
     if lower:
         err = np.tril(err)
@@ -87,8 +98,8 @@ def test_gemm(h, tier, ta, tb, m, n, k):
         R = alpha * oracle.dgemm(opA, opB) + beta * C0.astype(np.float64)
         S = abs(alpha) * absprod(opA, opB) + abs(beta) * np.abs(C0)
         u = U64 if tier == "fp64" else U32
-        extra = 3.0 + (2.0 ** 4 if tier == "split" else 0.0)   # 2^-20 = 16 * 2^-24
-        assert_bound(C, R, S, k, u, extra=extra, what=f"{tier} gemm")
+        rep = abs(alpha) * split_rep(opA, opB) if tier == "split" else 0.0
+        assert_bound(C, R, S, k, u, extra_abs=rep, what=f"{tier} gemm")
 
 
 # ------------------------------------------------------------------- SYRK
@@ -116,8 +127,8 @@ def test_syrk(h, tier, n, k):
         R = oracle.dsyrk(A, alpha, beta, C0)
         S = abs(alpha) * absprod(A, A.T) + abs(beta) * np.abs(C0)
         u = U64 if tier == "fp64" else U32
-        extra = 3.0 + (16.0 if tier == "split" else 0.0)
-        assert_bound(C, R, S, k, u, extra=extra, lower=True, what=f"{tier} syrk")
+        rep = abs(alpha) * split_rep(A, A.T) if tier == "split" else 0.0
+        assert_bound(C, R, S, k, u, lower=True, what=f"{tier} syrk", extra_abs=rep)
 
 
 def test_syrk_benchmark_size_plain(h):
@@ -158,8 +169,8 @@ def test_syr2k(h, tier, n, k, same_ld):
     R = oracle.dsyr2k(A, B, alpha, beta, C0)
     S = abs(alpha) * (absprod(A, B.T) + absprod(B, A.T)) + abs(beta) * np.abs(C0)
     u = U64 if tier == "fp64" else U32
-    extra = 3.0 + (16.0 if tier == "split" else 0.0)
-    assert_bound(C, R, S, 2 * k, u, extra=extra, lower=True, what=f"{tier} syr2k")
+    rep = abs(alpha) * (split_rep(A, B.T) + split_rep(B, A.T)) if tier == "split" else 0.0
+    assert_bound(C, R, S, 2 * k, u, lower=True, what=f"{tier} syr2k", extra_abs=rep)
 
 
 # ------------------------------------------------------------------- TRMM
@@ -182,8 +193,8 @@ def test_trmm(h, tier, m, n):
     R = oracle.dtrmm(L if tier != "plain" else Lt, B, -2.0)
     S = 2.0 * absprod(np.tril(Lt), B)
     u = U64 if tier == "fp64" else U32
-    extra = 3.0 + (16.0 if tier == "split" else 0.0)
-    assert_bound(C, R, S, m, u, extra=extra, what=f"{tier} trmm")
+    rep = 2.0 * split_rep(np.tril(Lt), B) if tier == "split" else 0.0
+    assert_bound(C, R, S, m, u, what=f"{tier} trmm", extra_abs=rep)
 
 
 # ------------------------------------------------------------------- SYMM
@@ -208,8 +219,8 @@ def test_symm(h, tier, m, n):
     R = oracle.dsymm(Asym, B, 1.0, 0.5, C0)
     S = absprod(Asym, B) + 0.5 * np.abs(C0)
     u = U64 if tier == "fp64" else U32
-    extra = 3.0 + (16.0 if tier == "split" else 0.0)
-    assert_bound(C, R, S, m, u, extra=extra, what=f"{tier} symm")
+    rep = split_rep(Asym, B) if tier == "split" else 0.0
+    assert_bound(C, R, S, m, u, what=f"{tier} symm", extra_abs=rep)
 
 
 # ------------------------------------------------------------------- TRSM
@@ -439,3 +450,80 @@ def test_error_codes(h, tbm):
         h.set_ozaki_slices(11)
     h.syrk("plain", 0, 10, 1.0, 0, 1, 0.0, 0, 1)              # empty: no-op
     h.synchronize()
+
+
+# --------------------------------------------- split tier with three FP16 planes
+# h + 2^-11 m + 2^-22 l is exact for FP32 inputs, so the three-plane tier keeps a
+# componentwise bound: FP32 accumulation plus 2^-20 = 16u for the products of weight
+# < 2^-22 it does not form.
+@pytest.fixture(scope="module")
+def h3(tbm):
+    hd = tbm.Handle()
+    hd.set_split_planes(3)
+    yield hd
+    hd.close()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 65, 19), (200, 131, 300), (256, 256, 1024)])
+def test_gemm_split3(h3, ta, tb, m, n, k):
+    g = rng(m * 3 + n * 5 + k + 7 * ta + 11 * tb)
+    A = uniform(g, (k, m) if ta else (m, k))
+    B = uniform(g, (n, k) if tb else (k, n))
+    C0 = uniform(g, (m, n))
+    dA, lda = to_dev(A, A.shape[0] + 1)
+    dB, ldb = to_dev(B, B.shape[0] + 2)
+    dC, ldc = to_dev(C0, m)
+    h3.gemm("split", ta, tb, m, n, k, 1.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, -0.5,
+            dC.data_ptr(), ldc)
+    h3.synchronize()
+    C = from_dev(dC, m, n, ldc)
+    opA, opB = (A.T if ta else A), (B.T if tb else B)
+    R = 1.5 * oracle.dgemm(opA, opB) - 0.5 * C0.astype(np.float64)
+    S = 1.5 * absprod(opA, opB) + 0.5 * np.abs(C0)
+    assert_bound(C, R, S, k, U32, extra=3.0 + 16.0, what="split3 gemm")
+
+
+@pytest.mark.parametrize("n,k", [(129, 64), (300, 257)])
+def test_syrk_split3(h3, n, k):
+    g = rng(n + 17 * k)
+    A = uniform(g, (n, k))
+    dA, lda = to_dev(A, n)
+    dC, ldc = to_dev(np.zeros((n, n), np.float32), n)
+    h3.syrk("split", n, k, 1.0, dA.data_ptr(), lda, 0.0, dC.data_ptr(), ldc, nb=256)
+    h3.synchronize()
+    C = from_dev(dC, n, n, ldc)
+    assert_bound(C, oracle.dsyrk(A), absprod(A, A.T), k, U32, extra=3.0 + 16.0, lower=True,
+                 what="split3 syrk")
+
+
+@pytest.mark.parametrize("m,n,nb", [(7, 300, 256), (64, 2048, 256), (33, 4100, 256)])
+def test_trsm_split3(h3, m, n, nb):
+    g = rng(m * 5 + n + nb)
+    L = lower_tri_dominant(g, n)
+    L[np.triu_indices(n, 1)] = np.nan
+    B = uniform(g, (m, n))
+    dL, lda = to_dev(L)
+    dB, ldb = to_dev(B, m + 1)
+    h3.trsm("split", m, n, dL.data_ptr(), lda, dB.data_ptr(), ldb, nb=nb)
+    h3.synchronize()
+    X = from_dev(dB, m, n, ldb)
+    e = oracle.errors(X, oracle.dtrsm_rltn(np.tril(np.nan_to_num(L)), B))
+    assert e["normwise"] < TRSM_TOL["split"] and e["maxrel"] < 4 * TRSM_TOL["split"], e
+
+
+def test_trsm_split3_vs_fp32(h3):
+    g = rng(0xC0F2)
+    m, n = 64, 4096
+    L = lower_tri_dominant(g, n)
+    B = uniform(g, (m, n))
+    R = oracle.dtrsm_rltn(L, B)
+    errs = {}
+    for tier in ("split", "fp32"):
+        dL, lda = to_dev(L)
+        dB, ldb = to_dev(B)
+        h3.trsm(tier, m, n, dL.data_ptr(), lda, dB.data_ptr(), ldb, nb=256)
+        h3.synchronize()
+        errs[tier] = oracle.errors(from_dev(dB, m, n, ldb), R)
+    assert errs["split"]["normwise"] <= 1.5 * errs["fp32"]["normwise"], errs
+    assert errs["split"]["maxrel"] <= 1.5 * errs["fp32"]["maxrel"], errs
