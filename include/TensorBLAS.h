@@ -135,6 +135,43 @@ TB_API int tb_symm(tb_handle_t h, int tier, long m, long n, double alpha, const 
 TB_API int tb_gemm(tb_handle_t h, int tier, int transa, int transb, long m, long n, long k, double alpha,
             const void* A, long lda, const void* B, long ldb, double beta, void* C, long ldc);
 
+/* ------------------------------------------------------------ batched plans
+ * Many independent SYRK and TRSM problems of mixed sizes, in one FP16 tier (PLAIN or
+ * SPLIT), planned once and then run as one HIP graph: the row exponents and the
+ * diagonal-block inverses of every instance (one launch each), then one persistent
+ * kernel whose workgroups take work items -- SYRK lower tiles, and 64-row TRSM panels
+ * that walk their instance's whole recursion -- from a queue sorted by decreasing
+ * work.  Each batched instance gives the same bits as its single tb_syrk / tb_trsm
+ * call in the same tier with the same nb.  The plan keeps the pointers it was given;
+ * the data behind them may change between runs (TRSM solves B in place on every run).
+ * No reference counterpart: the reference runs one problem per call. */
+typedef struct tb_batch_s* tb_batch_t;
+enum { TB_OP_SYRK = 0, TB_OP_TRSM = 1 };
+enum {
+    TB_BATCH_AUTOTUNE = 1,   /* time both SYRK tile shapes (64x256, 128x128) on scratch
+                                copies of the outputs and keep the faster */
+    TB_BATCH_NO_GRAPH = 2    /* launch the kernels directly on each run (no graph) */
+};
+typedef struct {
+    int op;              /* TB_OP_SYRK: C := alpha*A*A^T + beta*C, A n x k, lower C n x n
+                            TB_OP_TRSM: X*A^T = B in place, A = L n x n lower, B m x n */
+    long m, n, k;        /* SYRK: n, k (m unused);  TRSM: m, n (k unused) */
+    double alpha, beta;  /* SYRK only */
+    const float* A;      /* SYRK: A;  TRSM: L */
+    long lda;
+    float* C;            /* SYRK: C;  TRSM: B */
+    long ldc;
+} tb_batch_entry;
+TB_API int tb_batch_create(tb_handle_t handle, int tier, const tb_batch_entry* entries, int count,
+                    long nb, int flags, tb_batch_t* batch);
+/* one run of the plan on the handle's stream (asynchronous) */
+TB_API int tb_batch_run(tb_batch_t batch);
+/* out[0] SYRK tile shape in use (0: 64x256, 1: 128x128), out[1] work items, out[2]
+ * instances, out[3] / out[4] autotuning time of shape 0 / 1 in microseconds (0 when
+ * not tuned), out[5] device bytes of the plan */
+TB_API int tb_batch_info(tb_batch_t batch, long* out, int n);
+TB_API int tb_batch_destroy(tb_batch_t batch);
+
 /* ---------------------------------- conversion kernels (util/util.cu:127-167) */
 /* ah = fp16(as), m x n (RNE); the reference's s2h / s2h_swpipe */
 TB_API int tb_s2h(tb_handle_t h, long m, long n, const float* as, long ldas, void* ah, long ldah);

@@ -1,0 +1,104 @@
+// Device bodies shared by the single-call kernels (aux_kernels.hip) and the batched
+// kernels (batch.hip): row exponents and the inverses of TRSM diagonal blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tb_kernels.hpp"
+
+namespace tb {
+
+__device__ __forceinline__ int expo(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    int b = (u >> 23) & 0xff;
+    if (b) return b - 126;
+    if ((u & 0x7fffffffu) == 0) return kExpZero;
+    int e;
+    frexpf(x, &e);
+    return e;
+}
+
+__device__ __forceinline__ int expo(double x)
+{
+    uint64_t u = (uint64_t)__double_as_longlong(x);
+    int b = (int)((u >> 52) & 0x7ff);
+    if (b) return b - 1022;
+    if ((u & 0x7fffffffffffffffull) == 0) return kExpZero;
+    int e;
+    frexp(x, &e);
+    return e;
+}
+
+constexpr int kRowChunk = 256;    // k per block of the row-major pass
+constexpr int kColChunk = 1024;   // k per block of the k-contiguous pass
+
+// rows contiguous across threads: one thread per row of rows rb*256.., k chunk kb
+template <typename T>
+__device__ __forceinline__ void rowexp_mn_block(const T* __restrict__ X, long long ld, int rows,
+                                                int K, bool lower_only, int* out, int rb, int kb)
+{
+    const int r = rb * 256 + threadIdx.x;
+    if (r >= rows) return;
+    const int k0 = kb * kRowChunk;
+    int k1 = min(K, k0 + kRowChunk);
+    if (lower_only) k1 = min(k1, r + 1);
+    int e = kExpZero;
+    for (int k = k0; k < k1; ++k) e = max(e, expo(X[(long long)r + (long long)k * ld]));
+    if (e != kExpZero) atomicMax(out + r, e);
+}
+
+// k contiguous: one wave per row (rows rb*4..), lanes across k chunk kb; kfrom_diag
+// starts row r at k = r+1
+template <typename T>
+__device__ __forceinline__ void rowexp_k_block(const T* __restrict__ X, long long ld, int rows,
+                                               int K, bool kfrom_diag, int* out, int rb, int kb)
+{
+    const int r = rb * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= rows) return;
+    const T* Xr = X + (long long)r * ld;
+    int k0 = kb * kColChunk;
+    const int k1 = min(K, k0 + kColChunk);
+    if (kfrom_diag) k0 = max(k0, r + 1);
+    int e = kExpZero;
+    for (int k = k0 + lane; k < k1; k += 64) e = max(e, expo(Xr[k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
+    if (lane == 0 && e != kExpZero) atomicMax(out + r, e);
+}
+
+// Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.
+// Writes W = D inv(L) - I = (I+N)^-1 - I (strictly lower) as Z = W^T (Z[c + r*s] =
+// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One wave (blockDim 64) owns the
+// 64 columns cb.. of inv(L): lane c computes column cb+c by forward substitution,
+// keeping the column in LDS.  Row r of L is staged in LDS by the whole wave first,
+// so the inner product reads it as a broadcast.  y: 256*64 doubles, lrow: 256.
+template <typename Tin, typename Tout>
+__device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long long lda, int s,
+                                            Tout* __restrict__ Zq, double* __restrict__ dinvq,
+                                            int cb, double* y, double* lrow)
+{
+    const int lane = threadIdx.x;
+    const int col = cb + lane;
+    const bool live = col < s;
+    if (cb == 0)
+        for (int r = lane; r < s; r += 64) dinvq[r] = 1.0 / (double)Lq[r + (long long)r * lda];
+    for (int r = 0; r < cb; ++r)
+        if (live) Zq[col + (long long)r * s] = Tout(0);
+    for (int r = cb; r < s; ++r) {
+        for (int p = cb + lane; p <= r; p += 64) lrow[p - cb] = (double)Lq[r + (long long)p * lda];
+        __syncthreads();
+        double sum = (r == col) ? 1.0 : 0.0;
+        for (int p = cb; p < r; ++p) sum -= lrow[p - cb] * y[(p - cb) * 64 + lane];
+        const double d = lrow[r - cb];
+        const double v = (r < col || !live) ? 0.0 : sum / d;
+        y[(r - cb) * 64 + lane] = v;
+        // W(r, col) = L(r,r) * inv(L)(r, col) for r > col
+        if (live) Zq[col + (long long)r * s] = (r > col) ? (Tout)(d * v) : Tout(0);
+        __syncthreads();
+    }
+}
+
+}  // namespace tb

@@ -5,72 +5,29 @@
 
 #include <cstdint>
 
+#include "aux_device.hpp"
 #include "tb_kernels.hpp"
 
 namespace tb {
 
 namespace {
 
-__device__ __forceinline__ int expo(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    int b = (u >> 23) & 0xff;
-    if (b) return b - 126;
-    if ((u & 0x7fffffffu) == 0) return kExpZero;
-    int e;
-    frexpf(x, &e);
-    return e;
-}
-
-__device__ __forceinline__ int expo(double x)
-{
-    uint64_t u = (uint64_t)__double_as_longlong(x);
-    int b = (int)((u >> 52) & 0x7ff);
-    if (b) return b - 1022;
-    if ((u & 0x7fffffffffffffffull) == 0) return kExpZero;
-    int e;
-    frexp(x, &e);
-    return e;
-}
-
-constexpr int kRowChunk = 256;    // k per block of the row-major pass
-constexpr int kColChunk = 1024;   // k per block of the k-contiguous pass
-
-// rows contiguous across threads: one thread per row, k split over blockIdx.y
 template <typename T>
 __global__ void __launch_bounds__(256)
 rowexp_mn_kernel(const T* __restrict__ X, long long ld, int rows, int K, bool lower_only,
                  long long sx, int* out, long long so)
 {
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= rows) return;
-    const T* Xb = X + (long long)blockIdx.z * sx;
-    const int k0 = blockIdx.y * kRowChunk;
-    int k1 = min(K, k0 + kRowChunk);
-    if (lower_only) k1 = min(k1, r + 1);
-    int e = kExpZero;
-    for (int k = k0; k < k1; ++k) e = max(e, expo(Xb[(long long)r + (long long)k * ld]));
-    if (e != kExpZero) atomicMax(out + (long long)blockIdx.z * so + r, e);
+    rowexp_mn_block(X + (long long)blockIdx.z * sx, ld, rows, K, lower_only,
+                    out + (long long)blockIdx.z * so, blockIdx.x, blockIdx.y);
 }
 
-// k contiguous: one wave per row, lanes across k; kfrom_diag starts row r at k = r+1
 template <typename T>
 __global__ void __launch_bounds__(256)
 rowexp_k_kernel(const T* __restrict__ X, long long ld, int rows, int K, bool kfrom_diag,
                 long long sx, int* out, long long so)
 {
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (r >= rows) return;
-    const T* Xb = X + (long long)blockIdx.z * sx + (long long)r * ld;
-    int k0 = blockIdx.y * kColChunk;
-    const int k1 = min(K, k0 + kColChunk);
-    if (kfrom_diag) k0 = max(k0, r + 1);
-    int e = kExpZero;
-    for (int k = k0 + lane; k < k1; k += 64) e = max(e, expo(Xb[k]));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
-    if (lane == 0 && e != kExpZero) atomicMax(out + (long long)blockIdx.z * so + r, e);
+    rowexp_k_block(X + (long long)blockIdx.z * sx, ld, rows, K, kfrom_diag,
+                   out + (long long)blockIdx.z * so, blockIdx.x, blockIdx.y);
 }
 
 __global__ void fill_i32_kernel(int* p, long long n, int v)
@@ -78,12 +35,6 @@ __global__ void fill_i32_kernel(int* p, long long n, int v)
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) p[i] = v;
 }
 
-// Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.
-// Writes W = D inv(L) - I = (I+N)^-1 - I (strictly lower) as Z = W^T (Z[c + r*s] =
-// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One wave owns 64 columns of
-// inv(L): lane c computes column cb+c by forward substitution, keeping the column in
-// LDS.  Row r of L is staged in LDS by the whole wave first, so the inner product
-// reads it as a broadcast.
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(64)
 trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride, int s,
@@ -91,28 +42,8 @@ trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride
 {
     __shared__ double y[256 * 64];
     __shared__ double lrow[256];
-    const Tin* Lq = L + (long long)blockIdx.y * bstride;
-    Tout* Zq = Z + (long long)blockIdx.y * s * s;
-    const int cb = blockIdx.x * 64;
-    const int lane = threadIdx.x;
-    const int col = cb + lane;
-    const bool live = col < s;
-    if (blockIdx.x == 0)
-        for (int r = lane; r < s; r += 64) dinv[(long long)blockIdx.y * s + r] = 1.0 / (double)Lq[r + (long long)r * lda];
-    for (int r = 0; r < cb; ++r)
-        if (live) Zq[col + (long long)r * s] = Tout(0);
-    for (int r = cb; r < s; ++r) {
-        for (int p = cb + lane; p <= r; p += 64) lrow[p - cb] = (double)Lq[r + (long long)p * lda];
-        __syncthreads();
-        double sum = (r == col) ? 1.0 : 0.0;
-        for (int p = cb; p < r; ++p) sum -= lrow[p - cb] * y[(p - cb) * 64 + lane];
-        const double d = lrow[r - cb];
-        const double v = (r < col || !live) ? 0.0 : sum / d;
-        y[(r - cb) * 64 + lane] = v;
-        // W(r, col) = L(r,r) * inv(L)(r, col) for r > col
-        if (live) Zq[col + (long long)r * s] = (r > col) ? (Tout)(d * v) : Tout(0);
-        __syncthreads();
-    }
+    trtri_block(L + (long long)blockIdx.y * bstride, lda, s, Z + (long long)blockIdx.y * s * s,
+                dinv + (long long)blockIdx.y * s, blockIdx.x * 64, y, lrow);
 }
 
 // FP32 -> FP16, round to nearest even

@@ -236,18 +236,42 @@ struct Panel16 {
     }
 };
 
+// Compile-time shape of one FP16-MFMA tile configuration.
 template <int NPL, int SRCA, int SRCB, int BM, int BN, int WM, int WN, int NST>
-__global__ void __launch_bounds__(64 * WM * WN)
-gemm16_kernel(GemmLaunch L)
-{
-    constexpr int NT = 64 * WM * WN;
+struct Gemm16Cfg {
+    static constexpr int NT = 64 * WM * WN;
     using PA = Panel16<NPL, SRCA, BM, NT>;
     using PB = Panel16<NPL, SRCB, BN, NT>;
+    static constexpr int STAGE = PA::BYTES + PB::BYTES;
+    static constexpr int SMEM = NST * STAGE;        // LDS ring bytes
+    static constexpr bool kSplit = (NPL >= 2);     // 2 or 3 planes: split tier
+};
+
+// LDS a workgroup needs to run tiles of one configuration.
+template <class CFG, int BM, int BN>
+struct Gemm16Lds {
+    __attribute__((aligned(1024))) char ring[CFG::SMEM];
+    int scA[CFG::kSplit ? BM : 1];
+    int scB[CFG::kSplit ? BN : 1];
+    int emax[BM];
+};
+
+// One output tile C(i0:i0+BM, j0:j0+BN) of batch element bi of problem P, computed by
+// the whole workgroup.  Ends with a barrier, so a persistent caller may start the next
+// tile in the same LDS.
+template <int NPL, int SRCA, int SRCB, int BM, int BN, int WM, int WN, int NST>
+__device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, int j0,
+                                            char* smem, int* scA_s, int* scB_s, int* emax_s)
+{
+    using CFG = Gemm16Cfg<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>;
+    constexpr int NT = CFG::NT;
+    using PA = typename CFG::PA;
+    using PB = typename CFG::PB;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 32, FN = WTN / 32;
     static_assert(FM >= 1 && FN >= 1, "wave tile");
-    constexpr int STAGE = PA::BYTES + PB::BYTES;
-    constexpr bool kSplit = (NPL >= 2);     // 2 or 3 planes: split tier
+    constexpr int STAGE = CFG::STAGE;
+    constexpr bool kSplit = CFG::kSplit;
     constexpr bool kLo = (NPL == 3);        // third plane and accumulator
     // direct-to-LDS instructions each wave issues per stage (the vmcnt of one stage)
     constexpr int DMA_W = (SRCA == kSrcPlanes ? PA::DMA_PER_WAVE : 0) +
@@ -258,40 +282,9 @@ gemm16_kernel(GemmLaunch L)
     constexpr int KEEP = (NST - 2) * DMA_W;
     static_assert(KEEP <= 60, "vmcnt");
 
-    __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
-    __shared__ int scA_s[kSplit ? BM : 1];
-    __shared__ int scB_s[kSplit ? BN : 1];
-    __shared__ int emax_s[BM];
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (SGPR)
-    const int g = xcd_remap(blockIdx.x, L.totalTiles);
-    int pi = 0;
-#pragma unroll 1
-    for (int q = 1; q < L.np; ++q)
-        if (g >= L.p[q].tileBegin) pi = q;
-    const GemmProb& P = L.p[pi];
-
-    int local = g - P.tileBegin;
-    const int bi = local / P.tilesPer;
-    local -= bi * P.tilesPer;
-    int ti, tj;
-    if (P.modeC == kCLower) {
-        ti = (int)((__builtin_sqrtf(8.0f * (float)local + 1.0f) - 1.0f) * 0.5f);
-        while ((ti + 1) * (ti + 2) / 2 <= local) ++ti;
-        while (ti * (ti + 1) / 2 > local) --ti;
-        tj = local - ti * (ti + 1) / 2;
-    } else {
-        const int per_group = kGroupM * P.tilesN;
-        const int grp = local / per_group;
-        const int first = grp * kGroupM;
-        const int gsz = min(P.tilesM - first, kGroupM);
-        const int in = local - grp * per_group;
-        ti = first + in % gsz;
-        tj = in / gsz;
-    }
-    const int i0 = ti * BM, j0 = tj * BN;
     const int M = P.m, N = P.n, K = P.k;
 
     int kend = K;
@@ -533,6 +526,42 @@ gemm16_kernel(GemmLaunch L)
             if (emax_s[r] != kExpZeroDev && i0 + r < M)
                 atomicMax(P.eOut + (long long)bi * P.seA + i0 + r, emax_s[r]);
     }
+    __syncthreads();
+}
+
+template <int NPL, int SRCA, int SRCB, int BM, int BN, int WM, int WN, int NST>
+__global__ void __launch_bounds__(64 * WM * WN)
+gemm16_kernel(GemmLaunch L)
+{
+    using CFG = Gemm16Cfg<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>;
+    __shared__ Gemm16Lds<CFG, BM, BN> lds;
+    const int g = xcd_remap(blockIdx.x, L.totalTiles);
+    int pi = 0;
+#pragma unroll 1
+    for (int q = 1; q < L.np; ++q)
+        if (g >= L.p[q].tileBegin) pi = q;
+    const GemmProb& P = L.p[pi];
+
+    int local = g - P.tileBegin;
+    const int bi = local / P.tilesPer;
+    local -= bi * P.tilesPer;
+    int ti, tj;
+    if (P.modeC == kCLower) {
+        ti = (int)((__builtin_sqrtf(8.0f * (float)local + 1.0f) - 1.0f) * 0.5f);
+        while ((ti + 1) * (ti + 2) / 2 <= local) ++ti;
+        while (ti * (ti + 1) / 2 > local) --ti;
+        tj = local - ti * (ti + 1) / 2;
+    } else {
+        const int per_group = kGroupM * P.tilesN;
+        const int grp = local / per_group;
+        const int first = grp * kGroupM;
+        const int gsz = min(P.tilesM - first, kGroupM);
+        const int in = local - grp * per_group;
+        ti = first + in % gsz;
+        tj = in / gsz;
+    }
+    gemm16_tile<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>(P, bi, ti * BM, tj * BN, lds.ring, lds.scA,
+                                                      lds.scB, lds.emax);
 }
 
 }  // namespace tb

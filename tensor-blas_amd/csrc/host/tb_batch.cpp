@@ -1,0 +1,436 @@
+// Batch plans (tb_batch_*): many independent SYRK / TRSM instances of mixed sizes in one
+// HIP graph -- batched row exponents and diagonal-block inverses, then the persistent
+// batch kernel of batch.hip over a work list sorted by decreasing work.
+//
+// The plan reproduces the single calls exactly: the TRSM schedule comes from the same
+// planner (trsm_schedule, BLAS/tc_trsm.cu:31-137), the exponents from the same row
+// scans, the inverses from the same trtri body, and the tiles run the same MFMA
+// sequence per element.  tests/test_gpu_batch.py checks batch == single calls bitwise.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../tb_batch.hpp"
+#include "tb_internal.hpp"
+
+using namespace tb;
+
+struct tb_batch_s {
+    tb_handle_s* h = nullptr;
+    int tier = 0, unpl = 0, anpl = 0, sq = 0, flags = 0;
+    int count = 0, ninst_trsm = 0, ninst_syrk = 0;
+    char* dev = nullptr;
+    size_t dev_bytes = 0;
+    BatchItem* items[2] = {nullptr, nullptr};
+    int nitems[2] = {0, 0};
+    BatchInst* insts = nullptr;
+    BatchStep* steps = nullptr;
+    RowexpJob* rj[2] = {nullptr, nullptr};
+    int nrj[2] = {0, 0}, rjblocks[2] = {0, 0};
+    TrtriJob* tj = nullptr;
+    int ntj = 0, tjblocks = 0;
+    int* exps = nullptr;
+    long long nexps = 0;
+    int* counter = nullptr;
+    int grid = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double tune_us[2] = {0.0, 0.0};
+};
+
+namespace {
+
+bool dim_ok(long v) { return v >= 0 && v < (1l << 31); }
+
+int check_entry(const tb_batch_entry& e)
+{
+    if (e.op == TB_OP_SYRK) {
+        if (!dim_ok(e.n) || !dim_ok(e.k)) return TB_ERR_INVALID;
+        if (e.lda < std::max(1l, e.n) || e.ldc < std::max(1l, e.n)) return TB_ERR_INVALID;
+        if (e.n > 0 && (!e.C || (e.k > 0 && !e.A))) return TB_ERR_INVALID;
+        return TB_SUCCESS;
+    }
+    if (e.op == TB_OP_TRSM) {
+        if (!dim_ok(e.m) || !dim_ok(e.n)) return TB_ERR_INVALID;
+        if (e.lda < std::max(1l, e.n) || e.ldc < std::max(1l, e.m)) return TB_ERR_INVALID;
+        if (e.m > 0 && e.n > 0 && (!e.A || !e.C)) return TB_ERR_INVALID;
+        return TB_SUCCESS;
+    }
+    return TB_ERR_INVALID;
+}
+
+struct Costed {
+    BatchItem it;
+    double cost;
+};
+
+void sort_items(std::vector<Costed>& v)
+{
+    std::stable_sort(v.begin(), v.end(), [](const Costed& a, const Costed& b) { return a.cost > b.cost; });
+}
+
+// the launch sequence of one run on stream s (captured once into the plan's graph)
+hipError_t enqueue(const tb_batch_s* b, hipStream_t s)
+{
+    hipError_t e;
+    if (b->nexps > 0 && (e = fill_i32(b->exps, b->nexps, kExpZero, s)) != hipSuccess) return e;
+    if ((e = fill_i32(b->counter, 1, 0, s)) != hipSuccess) return e;
+    if ((e = batch_rowexp(b->rj[0], b->nrj[0], b->rjblocks[0], s)) != hipSuccess) return e;
+    if ((e = batch_trtri(b->tj, b->ntj, b->tjblocks, s)) != hipSuccess) return e;
+    if ((e = batch_rowexp(b->rj[1], b->nrj[1], b->rjblocks[1], s)) != hipSuccess) return e;
+    return batch_run(b->unpl, b->anpl, b->sq, b->items[b->sq], b->nitems[b->sq], b->insts, b->steps,
+                     b->counter, b->grid, s);
+}
+
+void release(tb_batch_s* b)
+{
+    // the plan may still run on its handle's stream; the handle itself may be gone
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->dev) (void)hipFree(b->dev);
+    delete b;
+}
+
+// Builds the device tables and workspace of a plan (no graph, no tuning).
+int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb, tb_batch_s* b)
+{
+    const bool split = tier == kSplitF16;
+    b->h = h;
+    b->tier = tier;
+    b->count = count;
+    b->unpl = planes_of(h, tier);
+    b->anpl = planes_of(h, kSplitF16);      // leaves run in the split tier (as single calls do)
+    b->grid = batch_grid(h->device);
+
+    std::vector<BatchInst> insts(count);
+    std::vector<BatchStep> steps;
+    std::vector<Costed> items[2];
+    std::vector<RowexpJob> rj[2];
+    std::vector<TrtriJob> tj;
+    // workspace offsets, fixed up to pointers after the allocation
+    struct Off {
+        long long eRow = -1, eSnap = -1, eL = -1, eZ = -1, Z = -1, dinv = -1;
+    };
+    std::vector<Off> off(count);
+    long long nexps = 0, nsnap = 0, nz = 0, ndinv = 0;
+
+    std::vector<TrsmStep> sched;
+    std::vector<TrsmLeaf> leaves;
+    for (int i = 0; i < count; ++i) {
+        const tb_batch_entry& e = E[i];
+        BatchInst& I = insts[i];
+        std::memset(&I, 0, sizeof(I));
+        I.op = e.op == TB_OP_SYRK ? kBatchSyrk : kBatchTrsm;
+        I.m = (int)e.m;
+        I.n = (int)e.n;
+        I.k = (int)e.k;
+        I.alpha = (float)e.alpha;
+        I.beta = (float)e.beta;
+        I.A = e.A;
+        I.lda = e.lda;
+        I.C = e.C;
+        I.ldc = e.ldc;
+        if (e.op == TB_OP_SYRK) {
+            ++b->ninst_syrk;
+            if (e.n == 0) continue;
+            if (split) {
+                off[i].eRow = nexps;
+                nexps += e.n;
+            }
+            // 128 x 128 lower tiles (variant 1) and 64 x 256 tiles meeting the lower part (0)
+            const long t1 = (e.n + 127) / 128;
+            for (long ti = 0; ti < t1; ++ti)
+                for (long tj = 0; tj <= ti; ++tj)
+                    items[1].push_back({{i, (int)(ti * 128), (int)(tj * 128), 0}, 128.0 * 128.0 * e.k});
+            const long t0 = (e.n + 63) / 64;
+            for (long ti = 0; ti < t0; ++ti) {
+                const long last = std::min(e.n - 1, ti * 64 + 63) / 256;
+                for (long tj = 0; tj <= last; ++tj)
+                    items[0].push_back({{i, (int)(ti * 64), (int)(tj * 256), 0}, 64.0 * 256.0 * e.k});
+            }
+            continue;
+        }
+        ++b->ninst_trsm;
+        if (e.m == 0 || e.n == 0) continue;
+        trsm_schedule(e.n, nb, &sched, &leaves);
+        I.step0 = (int)steps.size();
+        I.nsteps = (int)sched.size();
+        // leaf blocks: W^T offsets and exponent offsets in schedule order
+        std::vector<long long> zoff(leaves.size()), eoff(leaves.size());
+        long long z = 0, ez = 0;
+        for (size_t q = 0; q < leaves.size(); ++q) {
+            zoff[q] = z;
+            eoff[q] = ez;
+            z += leaves[q].s * leaves[q].s;
+            ez += leaves[q].s;
+        }
+        size_t lq = 0;
+        for (const TrsmStep& s : sched) {
+            BatchStep st;
+            std::memset(&st, 0, sizeof(st));
+            st.type = s.type == 0 ? 0 : (s.acc ? 2 : 1);
+            st.a = (int)s.a;
+            st.b = (int)s.b;
+            st.c = (int)s.c;
+            st.d = (int)s.d;
+            if (s.type == 0) {
+                st.zoff = zoff[lq];
+                st.eoff = (int)eoff[lq];
+                ++lq;
+            }
+            steps.push_back(st);
+        }
+        off[i].eRow = nexps;
+        nexps += e.m;
+        off[i].eL = nexps;
+        nexps += e.n;
+        off[i].eZ = nexps;
+        nexps += ez;
+        off[i].eSnap = nsnap;
+        nsnap += e.m;
+        off[i].Z = nz;
+        nz += z;
+        off[i].dinv = ndinv;
+        ndinv += e.n;
+        for (long r0 = 0; r0 < e.m; r0 += kPanelRows) {
+            const double rows = (double)std::min<long>(kPanelRows, e.m - r0);
+            const Costed c = {{i, (int)r0, 0, 0}, rows * (double)e.n * (double)e.n * 0.5};
+            items[0].push_back(c);
+            items[1].push_back(c);
+        }
+    }
+
+    // device layout
+    Arena ar;
+    const size_t oInst = ar.reserve(sizeof(BatchInst) * std::max(1, count));
+    const size_t oStep = ar.reserve(sizeof(BatchStep) * std::max<size_t>(1, steps.size()));
+    const size_t oIt0 = ar.reserve(sizeof(BatchItem) * std::max<size_t>(1, items[0].size()));
+    const size_t oIt1 = ar.reserve(sizeof(BatchItem) * std::max<size_t>(1, items[1].size()));
+    const size_t oCnt = ar.reserve(sizeof(int));
+    const size_t oExp = ar.reserve(sizeof(int) * std::max<long long>(1, nexps));
+    const size_t oSnap = ar.reserve(sizeof(int) * std::max<long long>(1, nsnap));
+    const size_t oZ = ar.reserve(sizeof(float) * std::max<long long>(1, nz));
+    const size_t oD = ar.reserve(sizeof(double) * std::max<long long>(1, ndinv));
+    // job tables are sized after the pointers are known; reserve their upper bounds
+    const size_t oRj0 = ar.reserve(sizeof(RowexpJob) * (2 * (size_t)count + 1));
+    const size_t oRj1 = ar.reserve(sizeof(RowexpJob) * (size_t)std::max<size_t>(1, steps.size() + 1));
+    const size_t oTj = ar.reserve(sizeof(TrtriJob) * (size_t)std::max<size_t>(1, steps.size() + 1));
+    if (hipMalloc(&b->dev, ar.total()) != hipSuccess) {
+        (void)hipGetLastError();
+        b->dev = nullptr;
+        return TB_ERR_NOMEM;
+    }
+    b->dev_bytes = ar.total();
+    char* d = b->dev;
+    b->insts = reinterpret_cast<BatchInst*>(d + oInst);
+    b->steps = reinterpret_cast<BatchStep*>(d + oStep);
+    b->items[0] = reinterpret_cast<BatchItem*>(d + oIt0);
+    b->items[1] = reinterpret_cast<BatchItem*>(d + oIt1);
+    b->counter = reinterpret_cast<int*>(d + oCnt);
+    b->exps = reinterpret_cast<int*>(d + oExp);
+    b->nexps = nexps;
+    int* snap = reinterpret_cast<int*>(d + oSnap);
+    float* Z = reinterpret_cast<float*>(d + oZ);
+    double* dinv = reinterpret_cast<double*>(d + oD);
+    b->rj[0] = reinterpret_cast<RowexpJob*>(d + oRj0);
+    b->rj[1] = reinterpret_cast<RowexpJob*>(d + oRj1);
+    b->tj = reinterpret_cast<TrtriJob*>(d + oTj);
+
+    auto add_rowexp = [](std::vector<RowexpJob>& v, int& blocks, const float* X, long long ld,
+                         int mode, long rows, long K, int* out) {
+        if (rows <= 0 || K <= 0) return;
+        RowexpJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.X = X;
+        j.ld = ld;
+        j.rows = (int)rows;
+        j.K = (int)K;
+        j.kcontig = mode == kOpT;
+        j.lower = mode == kOpTriL;
+        j.out = out;
+        j.rowBlocks = j.kcontig ? (int)((rows + 3) / 4) : (int)((rows + 255) / 256);
+        j.kBlocks = j.kcontig ? (int)((K + 1023) / 1024) : (int)((K + 255) / 256);
+        j.blockBegin = blocks;
+        blocks += j.rowBlocks * j.kBlocks;
+        v.push_back(j);
+    };
+    for (int i = 0; i < count; ++i) {
+        const tb_batch_entry& e = E[i];
+        BatchInst& I = insts[i];
+        const Off& o = off[i];
+        if (o.eRow >= 0) I.eRow = b->exps + o.eRow;
+        if (e.op == TB_OP_SYRK) {
+            if (o.eRow >= 0) add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpN, e.n, e.k, I.eRow);
+            continue;
+        }
+        if (e.m == 0 || e.n == 0) continue;
+        I.eSnap = snap + o.eSnap;
+        I.eL = b->exps + o.eL;
+        I.eZ = b->exps + o.eZ;
+        I.Z = Z + o.Z;
+        I.dinv = dinv + o.dinv;
+        // exponents of B's rows and L's rows (split updates), as trsm_run scans them
+        add_rowexp(rj[0], b->rjblocks[0], e.C, e.ldc, kOpN, e.m, e.n, I.eRow);
+        add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpTriL, e.n, e.n, (int*)I.eL);
+        // inverses: one job per run of equal adjacent leaves, as trsm_run launches them
+        trsm_schedule(e.n, nb, &sched, &leaves);
+        long long z = 0, ez = 0;
+        for (size_t q = 0; q < leaves.size();) {
+            size_t r = q + 1;
+            while (r < leaves.size() && leaves[r].s == leaves[q].s &&
+                   leaves[r].off == leaves[r - 1].off + leaves[q].s)
+                ++r;
+            const long s = leaves[q].s, lo = leaves[q].off;
+            TrtriJob t;
+            std::memset(&t, 0, sizeof(t));
+            t.L = e.A + lo * (e.lda + 1);
+            t.lda = e.lda;
+            t.bstride = s * (e.lda + 1);
+            t.s = (int)s;
+            t.count = (int)(r - q);
+            t.Z = Z + o.Z + z;
+            t.dinv = dinv + o.dinv + lo;
+            t.colBlocks = (int)((s + 63) / 64);
+            t.blockBegin = b->tjblocks;
+            b->tjblocks += t.colBlocks * t.count;
+            tj.push_back(t);
+            // exponents of the columns of the run's W^T blocks (stacked rows of Z^T)
+            add_rowexp(rj[1], b->rjblocks[1], Z + o.Z + z, s, kOpT, s * (long)(r - q), s,
+                       (int*)I.eZ + ez);
+            z += s * s * (long long)(r - q);
+            ez += s * (long long)(r - q);
+            q = r;
+        }
+    }
+    b->nrj[0] = (int)rj[0].size();
+    b->nrj[1] = (int)rj[1].size();
+    b->ntj = (int)tj.size();
+    if (rj[0].size() > 2 * (size_t)count + 1 || rj[1].size() > steps.size() + 1 ||
+        tj.size() > steps.size() + 1)
+        return TB_ERR_INVALID;   // cannot happen: each leaf run is one step at least
+
+    for (int v = 0; v < 2; ++v) {
+        sort_items(items[v]);
+        std::vector<BatchItem> flat(items[v].size());
+        for (size_t q = 0; q < flat.size(); ++q) flat[q] = items[v][q].it;
+        b->nitems[v] = (int)flat.size();
+        if (!flat.empty())
+            TB_HIP(hipMemcpy(b->items[v], flat.data(), sizeof(BatchItem) * flat.size(), hipMemcpyHostToDevice));
+    }
+    if (count > 0) TB_HIP(hipMemcpy(b->insts, insts.data(), sizeof(BatchInst) * count, hipMemcpyHostToDevice));
+    if (!steps.empty())
+        TB_HIP(hipMemcpy(b->steps, steps.data(), sizeof(BatchStep) * steps.size(), hipMemcpyHostToDevice));
+    for (int v = 0; v < 2; ++v)
+        if (!rj[v].empty())
+            TB_HIP(hipMemcpy(b->rj[v], rj[v].data(), sizeof(RowexpJob) * rj[v].size(), hipMemcpyHostToDevice));
+    if (!tj.empty()) TB_HIP(hipMemcpy(b->tj, tj.data(), sizeof(TrtriJob) * tj.size(), hipMemcpyHostToDevice));
+    return TB_SUCCESS;
+}
+
+// Times both SYRK tile shapes on scratch copies of the outputs (the real outputs are
+// not touched) and keeps the faster one.
+int autotune(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb, tb_batch_s* b)
+{
+    std::vector<tb_batch_entry> T(E, E + count);
+    std::vector<void*> scratch;
+    int st = TB_SUCCESS;
+    for (int i = 0; i < count && st == TB_SUCCESS; ++i) {
+        const long cols = E[i].op == TB_OP_SYRK ? E[i].n : E[i].n;
+        const size_t bytes = sizeof(float) * (size_t)E[i].ldc * (size_t)std::max(1l, cols);
+        if (!E[i].C || cols == 0) continue;
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            st = TB_ERR_NOMEM;
+            break;
+        }
+        scratch.push_back(p);
+        if (hipMemcpy(p, E[i].C, bytes, hipMemcpyDeviceToDevice) != hipSuccess) st = TB_ERR_DEVICE;
+        T[i].C = static_cast<float*>(p);
+    }
+    tb_batch_s* t = new (std::nothrow) tb_batch_s();
+    if (!t) st = TB_ERR_NOMEM;
+    if (st == TB_SUCCESS) st = build(h, tier, T.data(), count, nb, t);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st == TB_SUCCESS && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = TB_ERR_DEVICE;
+    for (int v = 0; v < 2 && st == TB_SUCCESS; ++v) {
+        t->sq = v;
+        float ms = 0.0f;
+        if (enqueue(t, h->stream) != hipSuccess || hipEventRecord(e0, h->stream) != hipSuccess ||
+            enqueue(t, h->stream) != hipSuccess || hipEventRecord(e1, h->stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            st = TB_ERR_DEVICE;
+        b->tune_us[v] = 1e3 * ms;
+    }
+    if (st == TB_SUCCESS) b->sq = b->tune_us[1] < b->tune_us[0] ? 1 : 0;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (t) release(t);
+    for (void* p : scratch) (void)hipFree(p);
+    return st;
+}
+
+}  // namespace
+
+int tb_batch_create(tb_handle_t h, int tier, const tb_batch_entry* entries, int count, long nb,
+                    int flags, tb_batch_t* batch)
+{
+    if (!h || !batch || count < 0 || (count > 0 && !entries) || nb <= 0) return TB_ERR_INVALID;
+    if (tier != TB_TIER_PLAIN && tier != TB_TIER_SPLIT) return TB_ERR_INVALID;
+    if (flags & ~(TB_BATCH_AUTOTUNE | TB_BATCH_NO_GRAPH)) return TB_ERR_INVALID;
+    for (int i = 0; i < count; ++i) TB_TRY(check_entry(entries[i]));
+    *batch = nullptr;
+    tb_batch_s* b = new (std::nothrow) tb_batch_s();
+    if (!b) return TB_ERR_NOMEM;
+    b->flags = flags;
+    int st = build(h, tier, entries, count, nb, b);
+    if (st == TB_SUCCESS && (flags & TB_BATCH_AUTOTUNE) && b->ninst_syrk > 0)
+        st = autotune(h, tier, entries, count, nb, b);
+    if (st == TB_SUCCESS && !(flags & TB_BATCH_NO_GRAPH)) {
+        if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            st = TB_ERR_DEVICE;
+        } else {
+            const hipError_t eq = enqueue(b, b->cap);
+            const hipError_t ec = hipStreamEndCapture(b->cap, &b->graph);
+            if (eq != hipSuccess || ec != hipSuccess ||
+                hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+                st = TB_ERR_DEVICE;
+        }
+    }
+    if (st != TB_SUCCESS) {
+        (void)hipGetLastError();
+        release(b);
+        return st;
+    }
+    *batch = b;
+    return TB_SUCCESS;
+}
+
+int tb_batch_run(tb_batch_t b)
+{
+    if (!b) return TB_ERR_INVALID;
+    if (b->exec) return hip_status(hipGraphLaunch(b->exec, b->h->stream));
+    return hip_status(enqueue(b, b->h->stream));
+}
+
+int tb_batch_info(tb_batch_t b, long* out, int n)
+{
+    if (!b || n < 0 || (n > 0 && !out)) return TB_ERR_INVALID;
+    const long v[6] = {b->sq, b->nitems[b->sq], b->count, (long)b->tune_us[0], (long)b->tune_us[1],
+                       (long)b->dev_bytes};
+    for (int i = 0; i < n && i < 6; ++i) out[i] = v[i];
+    return TB_SUCCESS;
+}
+
+int tb_batch_destroy(tb_batch_t b)
+{
+    if (!b) return TB_ERR_INVALID;
+    release(b);
+    return TB_SUCCESS;
+}

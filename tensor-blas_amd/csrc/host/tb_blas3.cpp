@@ -428,6 +428,7 @@ struct Leaf {
 
 struct PlanOp {
     long type, a, b, c, d;   // leaf: 0, off, s;  update: 1, c0, w, k0, kw
+    bool acc = false;        // update inside a wide leaf block: runs in the leaf tier
 };
 
 struct TrsmCtx {
@@ -482,11 +483,12 @@ struct TrsmCtx {
     }
 
     // B(:, c0:c0+w) -= B(:, k0:k0+kw) * A(c0:c0+w, k0:k0+kw)^T
-    int update(long c0, long w, long k0, long kw, int tier)
+    // in_leaf: an update inside a leaf block wider than the leaf kernel (acc tier)
+    int update(long c0, long w, long k0, long kw, int tier, bool in_leaf = false)
     {
         if (w <= 0 || kw <= 0) return TB_SUCCESS;
         if (collect) {
-            ops.push_back({1, c0, w, k0, kw});
+            ops.push_back({1, c0, w, k0, kw, in_leaf});
             return TB_SUCCESS;
         }
         if (tier == kOzakiI8)
@@ -510,6 +512,11 @@ struct TrsmCtx {
             // the solved block X1 is final: convert it once here rather than in every
             // tile column of the product
             TB_TRY(x_planes(tier, k0, kw, &p));
+        } else if (tier == kSplitF16) {
+            // FP32 operands: every tile scales X1's rows by eA, which this product's
+            // epilogues raise meanwhile; read a snapshot so all tiles agree
+            TB_HIP(hipMemcpyAsync(eX, eBX, sizeof(int) * m, hipMemcpyDeviceToDevice, h->stream));
+            p.eA = eX;
         }
         if (split_exps() && !f64) p.eOut = eBX;
         GemmQueue q(h, tier, kShapeSquare, p.srcA, p.srcB);
@@ -525,7 +532,7 @@ struct TrsmCtx {
     {
         if (collect) {
             leaves.push_back({off, s});
-            ops.push_back({0, off, s, 0, 0});
+            ops.push_back({0, off, s, 0, 0, false});
             return TB_SUCCESS;
         }
         const size_t q = next_leaf++;
@@ -553,7 +560,7 @@ struct TrsmCtx {
         if (s <= kLeafMax) return apply_leaf(off, s);
         const long hh = round_up(s / 2, 8);
         TB_TRY(leaf_block(off, hh));
-        TB_TRY(update(off + hh, s - hh, off, hh, acc_tier));
+        TB_TRY(update(off + hh, s - hh, off, hh, acc_tier, true));
         return leaf_block(off + hh, s - hh);
     }
 
@@ -737,6 +744,16 @@ int trsm(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void
     char* ws;
     TB_TRY(workspace_get(h, t.total, &ws));
     return trsm_run(c, t, ws);
+}
+
+void trsm_schedule(long n, long nb, std::vector<TrsmStep>* steps, std::vector<TrsmLeaf>* leaves)
+{
+    TrsmCtx c = make_ctx(nullptr, kSplitF16, 1, n, nullptr, n, nullptr, 1, nb);
+    trsm_plan(c);
+    steps->clear();
+    leaves->clear();
+    for (const PlanOp& o : c.ops) steps->push_back({o.type, o.a, o.b, o.c, o.d, o.acc});
+    for (const Leaf& l : c.leaves) leaves->push_back({l.off, l.s});
 }
 
 int plan_trsm(long n, long nb, long* out, int cap)

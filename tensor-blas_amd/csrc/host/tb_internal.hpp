@@ -114,6 +114,18 @@ int trsm_refined(tb_handle_s* h, int tier, long m, long n, const void* A, long l
                  long ldb, long nb);
 // schedule of TRSM (leaves and updates in execution order), for inspection
 int plan_trsm(long n, long nb, long* out, int cap);
+// The TRSM schedule of an n-column solve with block nb as the single call runs it
+// (BLAS/tc_trsm.cu:31-137): steps in execution order (type 0 leaf: a = off, b = s;
+// type 1 update: a = c0, b = w, c = k0, d = kw; acc: an update inside a leaf block
+// wider than the leaf kernel, run in the leaf tier) and the leaves in order.
+struct TrsmStep {
+    long type, a, b, c, d;
+    bool acc;
+};
+struct TrsmLeaf {
+    long off, s;
+};
+void trsm_schedule(long n, long nb, std::vector<TrsmStep>* steps, std::vector<TrsmLeaf>* leaves);
 int symm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, long lda,
          const void* B, long ldb, double beta, void* C, long ldc);
 int gemm(tb_handle_s* h, int tier, int transa, int transb, long m, long n, long k, double alpha,

@@ -60,6 +60,10 @@ def lib():
             "tb_set_profiling": [_P, _I],
             "tb_profile_read": [_P, _P, _I],
             "tb_plan_trsm": [_L, _L, _P, _I],
+            "tb_batch_create": [_P, _I, _P, _I, _L, _I, _P],
+            "tb_batch_run": [_P],
+            "tb_batch_info": [_P, _P, _I],
+            "tb_batch_destroy": [_P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -174,6 +178,58 @@ class Handle:
 
     def copy_lower_to_upper(self, f64, n, A, lda):
         _check("tb_copy_lower_to_upper", lib().tb_copy_lower_to_upper(self._h, int(f64), n, A, lda))
+
+
+# ------------------------------------------------------------------ batches
+class BatchEntry(ctypes.Structure):
+    """tb_batch_entry of include/TensorBLAS.h."""
+    _fields_ = [("op", ctypes.c_int), ("m", ctypes.c_long), ("n", ctypes.c_long),
+                ("k", ctypes.c_long), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_long), ("C", ctypes.c_void_p),
+                ("ldc", ctypes.c_long)]
+
+
+BATCH_OPS = {"syrk": 0, "trsm": 1}
+BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+
+
+class Batch:
+    """A batch plan: many SYRK / TRSM problems run as one HIP graph (tb_batch_*).
+
+    entries: dicts with op ('syrk' | 'trsm'), n, and m (trsm) or k (syrk), A, lda,
+    C, ldc (device addresses; for trsm A is L and C is B), alpha / beta (syrk)."""
+
+    def __init__(self, handle, tier, entries, nb=256, autotune=False, graph=True):
+        arr = (BatchEntry * max(1, len(entries)))()
+        for i, e in enumerate(entries):
+            arr[i] = BatchEntry(BATCH_OPS[e["op"]], e.get("m", 0), e["n"], e.get("k", 0),
+                                e.get("alpha", 1.0), e.get("beta", 0.0), e["A"], e["lda"], e["C"],
+                                e["ldc"])
+        flags = (BATCH_AUTOTUNE if autotune else 0) | (0 if graph else BATCH_NO_GRAPH)
+        self._b = ctypes.c_void_p()
+        self._handle = handle          # the plan runs on the handle's stream
+        _check("tb_batch_create", lib().tb_batch_create(handle.ptr, tier_id(tier), arr, len(entries),
+                                                        nb, flags, ctypes.byref(self._b)))
+
+    def run(self):
+        _check("tb_batch_run", lib().tb_batch_run(self._b))
+
+    def info(self):
+        out = (ctypes.c_long * 6)()
+        _check("tb_batch_info", lib().tb_batch_info(self._b, out, 6))
+        return {"syrk_tile": "128x128" if out[0] else "64x256", "items": out[1],
+                "instances": out[2], "tune_us": (out[3], out[4]), "device_bytes": out[5]}
+
+    def close(self):
+        if self._b:
+            _check("tb_batch_destroy", lib().tb_batch_destroy(self._b))
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --------------------------------------------------------- host-only planners
