@@ -15,6 +15,9 @@ Other workloads (--workload):
          tiers beside it), error vs FP64 dsyrk, host OpenBLAS ssyrk baseline
   ozaki  FP64 SYR2K by Ozaki INT8 slicing, n = 32768, k = 8192, against the native
          FP64-MFMA tier; error vs an extended-precision host product on sampled columns
+  batched  2000 independent SYRK / TRSM instances (n log-uniform in [64, 2048]) as
+         one batch plan (persistent kernel, one HIP graph), against the same instances
+         as single calls in the same tier and in the FP32-FMA tier
   gemm   tb_gemm microbenchmark (--m --n --k --tiers)
   copy   a 256 MiB device copy (reference for HBM byte counters)
 
@@ -357,6 +360,141 @@ def run_ozaki(args, torch, tensorblas):
     return line
 
 
+# -------------------------------------------------------------------- batched
+def batched_instances(count, seed=0xC0F6):
+    """count instances: half SYRK (k = n), half TRSM (m = n), n log-uniform in
+    [64, 2048] rounded to an integer, shuffled."""
+    g = np.random.default_rng(seed)
+    ns = np.rint(np.exp(g.uniform(np.log(64), np.log(2048), count))).astype(np.int64)
+    ops = np.array(["syrk"] * (count // 2) + ["trsm"] * (count - count // 2))
+    g.shuffle(ops)
+    return list(zip(ops.tolist(), ns.tolist()))
+
+
+def run_batched(args, torch, tensorblas):
+    count = args.count or 2000
+    tier = args.tier or "split"
+    inst = batched_instances(count)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0xC0F6)
+    nin = sum(n * n for _, n in inst)
+    nb_out = sum(n * n for op, n in inst if op == "trsm")
+    nc_out = sum(n * n for op, n in inst if op == "syrk")
+    log(f"bench: batched {count} instances, {4 * (nin + 2 * nb_out + nc_out) / 2**30:.1f} GiB")
+    ins = torch.rand(nin, device="cuda", generator=gen) * 2 - 1        # A (SYRK) / L (TRSM)
+    bout = torch.rand(max(1, nb_out), device="cuda", generator=gen) * 2 - 1   # B (TRSM)
+    bsave = bout.clone()
+    cout = torch.zeros(max(1, nc_out), device="cuda")                # C (SYRK, beta = 0)
+    entries, views = [], []
+    oi = ob = oc = 0
+    flops = 0.0
+    for op, n in inst:
+        a = ins[oi:oi + n * n]
+        oi += n * n
+        if op == "trsm":
+            L = a.view(n, n)          # column-major L: row j of this view is column j
+            L.diagonal().copy_(n + torch.rand(n, device="cuda", generator=gen))
+            o = bout[ob:ob + n * n]
+            ob += n * n
+            entries.append(dict(op="trsm", m=n, n=n, A=a.data_ptr(), lda=n, C=o.data_ptr(), ldc=n))
+        else:
+            o = cout[oc:oc + n * n]
+            oc += n * n
+            entries.append(dict(op="syrk", n=n, k=n, A=a.data_ptr(), lda=n, C=o.data_ptr(), ldc=n,
+                                alpha=1.0, beta=0.0))
+        views.append((op, n, a, o))
+        flops += float(n) ** 3
+    h = tensorblas.Handle()
+    h.set_split_planes(args.split_planes)
+    t0 = time.perf_counter()
+    plan = tensorblas.Batch(h, tier, entries, nb=args.nb, autotune=True)
+    plan_s = time.perf_counter() - t0
+    info = plan.info()
+    log(f"bench: plan {plan_s:.2f} s, {info}")
+
+    def step():
+        bout.copy_(bsave)
+        plan.run()
+    ms = timed(h, step, args.steps, args.warmup)
+    prof = profiled(h, step)
+    roof, rk = roofline(prof, FP16_PEAK_TF, "TFLOP/s", "mfma", f"batched{count}",
+                        f"one graph run: batched exponent and inverse prologues + the persistent "
+                        f"kernel; {tier} tier")
+
+    # errors of sampled instances against FP64 host BLAS (outputs of the last run)
+    import scipy.linalg.blas as sb
+    errs = []
+    pick = [i for i, (op, n) in enumerate(inst) if op == "syrk"][:3] + \
+           [i for i, (op, n) in enumerate(inst) if op == "trsm"][:3]
+    for i in pick:
+        op, n, a, o = views[i]
+        A = a.view(n, n).cpu().numpy().T.astype(np.float64)
+        got = o.view(n, n).cpu().numpy().T
+        if op == "syrk":
+            R = sb.dsyrk(1.0, np.asfortranarray(A), lower=1)
+            lo = np.tril_indices(n)
+            errs.append(dict(op=op, n=n, **err_metrics(got[lo], R[lo])))
+        else:
+            j0 = sum(m * m for p, m in inst[:i] if p == "trsm")
+            B0 = bsave[j0:j0 + n * n].view(n, n).cpu().numpy().T.astype(np.float64)
+            R = sb.dtrsm(1.0, np.asfortranarray(np.tril(A)), np.asfortranarray(B0), side=1,
+                         lower=1, trans_a=1)
+            errs.append(dict(op=op, n=n, **err_metrics(got, R)))
+
+    # the same instances as one single call each (same tier; and the FP32-FMA tier)
+    def loop(t):
+        def f():
+            bout.copy_(bsave)
+            for (op, n, a, o), e in zip(views, entries):
+                if op == "syrk":
+                    h.syrk(t, n, n, 1.0, e["A"], n, 0.0, e["C"], n, nb=args.nb)
+                else:
+                    h.trsm(t, n, n, e["A"], n, e["C"], n, nb=args.nb)
+        return f
+    ms_loop = timed(h, loop(tier), 1, 1)
+    ms_fp32 = timed(h, loop("fp32"), 1, 1)
+    line = line_base("eff. TFLOP/s of a batched mixed SYRK+TRSM plan (one graph launch)",
+                     flops / (ms * 1e-3) / 1e12, "TFLOP/s", args.steps, args.warmup, ms,
+                     f"f32 ({tier} tier, {args.split_planes}-plane fp16 split MFMA)" if tier == "split"
+                     else "f16 in, f32 accumulate",
+                     "synthetic: A, B U[-1,1]; L U[-1,1] with diag n+U[0,1] (seeded, on device)",
+                     {"workload": f"{count} independent instances, {count // 2} SYRK (k=n) + "
+                                  f"{count - count // 2} TRSM (m=n), n log-uniform [64, 2048]",
+                      "count": count, "nb": args.nb, "tier": tier, "flops": flops})
+    line.update({
+        "plan": {"seconds": round(plan_s, 3), **info},
+        "vs_single_calls": {"ms": round(ms_loop, 3), "tflops": round(flops / (ms_loop * 1e-3) / 1e12, 2),
+                            "speedup": round(ms_loop / ms, 3)},
+        "vs_plain_fp32_fma": {"ms": round(ms_fp32, 3),
+                              "tflops": round(flops / (ms_fp32 * 1e-3) / 1e12, 2),
+                              "ratio": round(ms_fp32 / ms, 3)},
+        "errors_vs_fp64": errs, "roofline": roof, "roofline_kernel": rk,
+        "note": "each step restores the TRSM right-hand sides (device copy) before the run",
+    })
+    if not args.no_cpu:
+        ts, fl, done = 0.0, 0.0, 0
+        for (op, n, a, o), e in zip(views, entries):
+            A = np.asfortranarray(a.view(n, n).cpu().numpy().T)
+            t0 = time.perf_counter()
+            if op == "syrk":
+                sb.ssyrk(1.0, A, lower=1)
+            else:
+                sb.strsm(1.0, A, np.asfortranarray(o.view(n, n).cpu().numpy().T), side=1, lower=1,
+                         trans_a=1)
+            ts += time.perf_counter() - t0
+            fl += float(n) ** 3
+            done += 1
+            if ts > 10.0:
+                break
+        line["cpu_baseline"] = {"value": round(fl / ts / 1e12, 4), "unit": "TFLOP/s",
+                                "cores": int(host_threads()), "kind": "port",
+                                "impl": "host OpenBLAS ssyrk / strsm via scipy.linalg.blas",
+                                "sample": f"first {done} of {count} instances ({ts:.1f} s)"}
+    plan.close()
+    h.close()
+    return line
+
+
 # ----------------------------------------------------------------- microbench
 def run_gemm(args, torch, tensorblas):
     M, N, K = args.m or 8192, args.n or 8192, args.k or 8192
@@ -407,7 +545,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="trsm", choices=["trsm", "syrk", "ozaki", "gemm", "copy"])
+    ap.add_argument("--workload", default="trsm",
+                    choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched"])
+    ap.add_argument("--count", type=int, default=0)
+    ap.add_argument("--tier", default="")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--m", type=int, default=0)
     ap.add_argument("--k", type=int, default=0)
@@ -424,7 +565,7 @@ def main():
     import torch
     import tensorblas
     run = {"trsm": run_trsm, "syrk": run_syrk, "ozaki": run_ozaki, "gemm": run_gemm,
-           "copy": run_copy}[args.workload]
+           "copy": run_copy, "batched": run_batched}[args.workload]
     print(json.dumps(run(args, torch, tensorblas)), flush=True)
 
 

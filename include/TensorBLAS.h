@@ -92,8 +92,9 @@ TB_API long tb_workspace_bytes(tb_handle_t handle);
  * out[3c+2] = summed work: algorithmic flops, except category 13 (bytes read and
  * written).  Categories: 0..9 grouped GEMM (tier*2 + tile shape, shape 0 = 128x128
  * updates, 1 = 64x256 TRSM leaves), 10 Ozaki slice GEMM, 11 Ozaki slicing,
- * 12 diagonal-block inversion, 13 FP32 -> FP16 plane conversion of operands. */
-#define TB_PROF_CATEGORIES 14
+ * 12 diagonal-block inversion, 13 FP32 -> FP16 plane conversion of operands, 14 one
+ * run of a batch plan (tb_batch_run: prologue and persistent kernel). */
+#define TB_PROF_CATEGORIES 15
 TB_API int tb_set_profiling(tb_handle_t handle, int on);
 TB_API int tb_profile_read(tb_handle_t handle, double* out, int ncat);
 /* human-readable text of a status code */

@@ -38,6 +38,7 @@ struct tb_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double tune_us[2] = {0.0, 0.0};
+    double flops = 0.0;          // algorithmic flops of one run
 };
 
 namespace {
@@ -134,6 +135,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         I.lda = e.lda;
         I.C = e.C;
         I.ldc = e.ldc;
+        b->flops += e.op == TB_OP_SYRK ? (double)e.n * e.n * e.k : (double)e.m * e.n * e.n;
         if (e.op == TB_OP_SYRK) {
             ++b->ninst_syrk;
             if (e.n == 0) continue;
@@ -415,8 +417,11 @@ int tb_batch_create(tb_handle_t h, int tier, const tb_batch_entry* entries, int 
 int tb_batch_run(tb_batch_t b)
 {
     if (!b) return TB_ERR_INVALID;
-    if (b->exec) return hip_status(hipGraphLaunch(b->exec, b->h->stream));
-    return hip_status(enqueue(b, b->h->stream));
+    hipEvent_t e0;
+    TB_TRY(prof_begin(b->h, &e0));
+    if (b->exec) TB_HIP(hipGraphLaunch(b->exec, b->h->stream));
+    else TB_HIP(enqueue(b, b->h->stream));
+    return prof_end(b->h, kProfBatch, e0, b->flops);
 }
 
 int tb_batch_info(tb_batch_t b, long* out, int n)

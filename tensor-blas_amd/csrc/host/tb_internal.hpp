@@ -39,7 +39,8 @@ enum ProfCat : int {
     kProfOzakiSlice = 11,
     kProfTrtri = 12,
     kProfPlanes = 13,          // FP32 -> FP16 plane conversion (work = bytes moved)
-    kProfCats = 14,
+    kProfBatch = 14,           // one run of a batch plan (work = its algorithmic flops)
+    kProfCats = 15,
 };
 // Wraps one launch in events when the handle profiles.
 int prof_begin(tb_handle_s* h, hipEvent_t* e0);

@@ -116,7 +116,7 @@ class Handle:
 
     PROF_NAMES = ["gemm_plain_sq", "gemm_plain_leaf", "gemm_split_sq", "gemm_split_leaf",
                   "gemm_ozaki_sq", "gemm_ozaki_leaf", "gemm_fp32_sq", "gemm_fp32_leaf",
-                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri", "planes"]
+                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri", "planes", "batch"]
 
     def set_profiling(self, on):
         _check("tb_set_profiling", lib().tb_set_profiling(self._h, int(on)))
