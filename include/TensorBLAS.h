@@ -173,6 +173,43 @@ TB_API int tb_batch_run(tb_batch_t batch);
 TB_API int tb_batch_info(tb_batch_t batch, long* out, int n);
 TB_API int tb_batch_destroy(tb_batch_t batch);
 
+/* ------------------------------------------------------------- request loop
+ * tb_serve runs a stream of requests -- any routine, any tier, each with its own
+ * arrival time -- on nslots handles (each with its own stream and workspace) within
+ * an HBM budget.  A request is admitted first-come first-served as soon as a slot is
+ * free and its bytes fit next to those already admitted; admitted requests run
+ * concurrently on the device.  Operands are device pointers the caller keeps
+ * resident.  Times are device times (HIP events) in microseconds from the start of
+ * the stream; the call returns when every request has finished.  A request whose
+ * bytes exceed the budget is refused with TB_ERR_NOMEM.  Each request computes what
+ * the same call made alone computes (the kernels are deterministic).
+ * No reference counterpart. */
+enum { TB_REQ_SYRK = 0, TB_REQ_SYR2K = 1, TB_REQ_TRMM = 2, TB_REQ_TRSM = 3, TB_REQ_SYMM = 4 };
+typedef struct {
+    int op, tier;
+    long m, n, k, nb;      /* the routine's dimensions (SYRK/SYR2K: n, k; TRMM/TRSM/SYMM: m, n) */
+    double alpha, beta;
+    const void* A;         /* TRMM/TRSM: L */
+    long lda;
+    const void* B;         /* SYR2K, TRMM, SYMM */
+    long ldb;
+    void* C;               /* TRSM: B, solved in place */
+    long ldc;
+    double arrival_us;     /* from the start of the stream */
+    long bytes;            /* HBM held while admitted; 0: twice the operand bytes */
+} tb_request;
+typedef struct {
+    double admit_us, done_us;
+    int slot, status;      /* status of the routine call (TB_ERR_NOMEM: refused) */
+} tb_request_result;
+TB_API int tb_serve(tb_handle_t* slots, int nslots, long hbm_budget, const tb_request* reqs, int count,
+             tb_request_result* results);
+/* Grow a handle's two device workspaces (routines; Ozaki products) to at least these
+ * sizes, so that later calls never reallocate (which synchronizes the handle's stream);
+ * tb_workspace_sizes reads the current sizes into out[0], out[1]. */
+TB_API int tb_reserve_workspace(tb_handle_t handle, long bytes, long ozaki_bytes);
+TB_API int tb_workspace_sizes(tb_handle_t handle, long* out);
+
 /* ---------------------------------- conversion kernels (util/util.cu:127-167) */
 /* ah = fp16(as), m x n (RNE); the reference's s2h / s2h_swpipe */
 TB_API int tb_s2h(tb_handle_t h, long m, long n, const float* as, long ldas, void* ah, long ldah);

@@ -64,6 +64,9 @@ def lib():
             "tb_batch_run": [_P],
             "tb_batch_info": [_P, _P, _I],
             "tb_batch_destroy": [_P],
+            "tb_serve": [_P, _I, _L, _P, _I, _P],
+            "tb_reserve_workspace": [_P, _L, _L],
+            "tb_workspace_sizes": [_P, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -176,6 +179,14 @@ class Handle:
     def s2h_transpose(self, m, n, As, Ah):
         _check("tb_s2h_transpose", lib().tb_s2h_transpose(self._h, m, n, As, Ah))
 
+    def reserve_workspace(self, nbytes, ozaki_bytes=0):
+        _check("tb_reserve_workspace", lib().tb_reserve_workspace(self._h, nbytes, ozaki_bytes))
+
+    def workspace_sizes(self):
+        out = (ctypes.c_long * 2)()
+        _check("tb_workspace_sizes", lib().tb_workspace_sizes(self._h, out))
+        return int(out[0]), int(out[1])
+
     def copy_lower_to_upper(self, f64, n, A, lda):
         _check("tb_copy_lower_to_upper", lib().tb_copy_lower_to_upper(self._h, int(f64), n, A, lda))
 
@@ -230,6 +241,43 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- request loop
+class Request(ctypes.Structure):
+    """tb_request of include/TensorBLAS.h."""
+    _fields_ = [("op", ctypes.c_int), ("tier", ctypes.c_int), ("m", ctypes.c_long),
+                ("n", ctypes.c_long), ("k", ctypes.c_long), ("nb", ctypes.c_long),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("A", ctypes.c_void_p),
+                ("lda", ctypes.c_long), ("B", ctypes.c_void_p), ("ldb", ctypes.c_long),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_long), ("arrival_us", ctypes.c_double),
+                ("bytes", ctypes.c_long)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("admit_us", ctypes.c_double), ("done_us", ctypes.c_double),
+                ("slot", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+REQ_OPS = {"syrk": 0, "syr2k": 1, "trmm": 2, "trsm": 3, "symm": 4}
+
+
+def serve(slots, budget_bytes, requests):
+    """Run a request stream (tb_serve) on the given handles; requests are dicts with
+    op, tier, m, n, k, nb, alpha, beta, A, lda, B, ldb, C, ldc, arrival_us, bytes.
+    Returns a list of dicts admit_us, done_us, slot, status."""
+    n = len(requests)
+    arr = (Request * max(1, n))()
+    for i, r in enumerate(requests):
+        arr[i] = Request(REQ_OPS[r["op"]], tier_id(r["tier"]), r.get("m", 0), r["n"], r.get("k", 0),
+                         r.get("nb", 256), r.get("alpha", 1.0), r.get("beta", 0.0), r["A"],
+                         r["lda"], r.get("B", 0) or 0, r.get("ldb", 1), r["C"], r["ldc"],
+                         float(r.get("arrival_us", 0.0)), int(r.get("bytes", 0)))
+    res = (RequestResult * max(1, n))()
+    hs = (ctypes.c_void_p * len(slots))(*[s.ptr.value for s in slots])
+    _check("tb_serve", lib().tb_serve(hs, len(slots), int(budget_bytes), arr, n, res))
+    return [dict(admit_us=res[i].admit_us, done_us=res[i].done_us, slot=res[i].slot,
+                 status=res[i].status) for i in range(n)]
 
 
 # --------------------------------------------------------- host-only planners

@@ -1,0 +1,113 @@
+"""The request loop (tb_serve): mixed routines and tiers on concurrent slots.
+
+Each request must compute exactly what it computes alone (bitwise: the kernels are
+deterministic and slots share nothing but the device); admission must respect the
+HBM budget and first-come first-served order."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def tbm():
+    import tensorblas
+    return tensorblas
+
+
+def make_requests(seed, sizes=(96, 200, 320)):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    reqs, keep = [], []
+    ops = ["syrk", "syr2k", "trmm", "trsm", "symm"]
+    q = 0
+    for tier in ("plain", "split", "ozaki"):
+        dt = torch.float64 if tier == "ozaki" else torch.float32
+        for op in ops:
+            n = sizes[q % len(sizes)]
+            q += 1
+
+            def rnd(*shape):
+                return (torch.rand(*shape, device="cuda", dtype=dt, generator=g) * 2 - 1).contiguous()
+            r = dict(op=op, tier=tier, nb=256, alpha=1.0, beta=0.0)
+            if op in ("syrk", "syr2k"):
+                A, B, C = rnd(n, n), rnd(n, n), rnd(n, n)     # column-major n x k (k = n)
+                r.update(n=n, k=n, A=A.data_ptr(), lda=n, B=B.data_ptr(), ldb=n, C=C.data_ptr(),
+                         ldc=n, beta=0.5)
+            elif op == "trsm":
+                L, B = rnd(n, n), rnd(n, n)
+                L.diagonal().copy_(n + torch.rand(n, device="cuda", dtype=dt, generator=g))
+                A, C = L, B
+                r.update(m=n, n=n, A=L.data_ptr(), lda=n, C=B.data_ptr(), ldc=n)
+            else:
+                A, B, C = rnd(n, n), rnd(n, n), rnd(n, n)
+                r.update(m=n, n=n, A=A.data_ptr(), lda=n, B=B.data_ptr(), ldb=n, C=C.data_ptr(),
+                         ldc=n, beta=0.25 if op == "symm" else 0.0)
+            reqs.append(r)
+            keep.append((A, B, C, C.clone()))       # operands and the output's initial value
+    return reqs, keep
+
+
+def restore(keep):
+    for _, _, C, C0 in keep:
+        C.copy_(C0)
+    torch.cuda.synchronize()
+
+
+def slots(tbm, n):
+    hs = []
+    for _ in range(n):
+        h = tbm.Handle(torch.cuda.Stream())
+        h.reserve_workspace(64 << 20, 64 << 20)
+        hs.append(h)
+    return hs
+
+
+def test_serve_mix_equals_alone(tbm):
+    reqs, keep = make_requests(0xC0F7)
+    alone = []
+    h1 = slots(tbm, 1)
+    for r, (_, _, C, _) in zip(reqs, keep):
+        res = tbm.serve(h1, 1 << 34, [dict(r, arrival_us=0.0)])
+        assert res[0]["status"] == 0 and res[0]["done_us"] > 0, r
+        alone.append(C.clone())
+    restore(keep)
+    hs = slots(tbm, 4)
+    # all at once (maximal overlap), then staggered arrivals
+    for spacing in (0.0, 50.0):
+        restore(keep)
+        mix = [dict(r, arrival_us=spacing * i) for i, r in enumerate(reqs)]
+        res = tbm.serve(hs, 1 << 34, mix)
+        for i, (r, x) in enumerate(zip(res, reqs)):
+            assert r["status"] == 0 and r["done_us"] >= r["admit_us"] - 50.0, (i, r)
+            assert torch.equal(keep[i][2], alone[i]), (spacing, x["op"], x["tier"], x["n"])
+        assert len({r["slot"] for r in res}) > 1           # the work really was spread
+
+
+def test_serve_budget_and_order(tbm):
+    reqs, keep = make_requests(3, sizes=(128,))
+    hs = slots(tbm, 4)
+    # a budget that holds one request at a time: strictly sequential, in arrival order
+    one = [dict(r, bytes=1000, arrival_us=0.0) for r in reqs]
+    res = tbm.serve(hs, 1500, one)
+    assert all(r["status"] == 0 for r in res)
+    for a, b in zip(res, res[1:]):
+        assert b["admit_us"] >= a["admit_us"]
+        assert b["admit_us"] + 1.0 >= a["done_us"] - 200.0     # device vs host clock slack
+    # a request larger than the budget is refused, the others run
+    big = [dict(r, bytes=1000, arrival_us=0.0) for r in reqs[:3]]
+    big[1]["bytes"] = 10 ** 9
+    res = tbm.serve(hs, 5000, big)
+    assert [r["status"] for r in res] == [0, 3, 0]
+
+
+def test_serve_rejects_bad_arguments(tbm):
+    hs = slots(tbm, 1)
+    with pytest.raises(tbm.TensorBLASError):
+        tbm.serve(hs, 0, [])
+    z = torch.zeros(4, device="cuda")
+    with pytest.raises(tbm.TensorBLASError):
+        tbm.serve(hs, 1 << 20, [dict(op="syrk", tier="plain", n=1, k=1, A=z.data_ptr(), lda=1,
+                                     C=z.data_ptr(), ldc=1, arrival_us=-1.0)])
