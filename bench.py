@@ -18,6 +18,11 @@ Other workloads (--workload):
   batched  2000 independent SYRK / TRSM instances (n log-uniform in [64, 2048]) as
          one batch plan (persistent kernel, one HIP graph), against the same instances
          as single calls in the same tier and in the FP32-FMA tier
+  serve  a stream of mixed requests (SYRK 40 / TRSM 30 / TRMM 15 / SYMM 10 / SYR2K 5 %,
+         n log-uniform [128, 8192], tiers plain / split / ozaki) with Poisson arrivals at
+         --load of the offline capacity, through the request loop on --slots streams:
+         p50/p99 latency against isolated calls, throughput against offline, results
+         checked equal to the isolated calls
   gemm   tb_gemm microbenchmark (--m --n --k --tiers)
   copy   a 256 MiB device copy (reference for HBM byte counters)
 
@@ -495,6 +500,126 @@ def run_batched(args, torch, tensorblas):
     return line
 
 
+# ---------------------------------------------------------------------- serve
+SERVE_MIX = [("syrk", 0.40), ("trsm", 0.30), ("trmm", 0.15), ("symm", 0.10), ("syr2k", 0.05)]
+SERVE_FLOPS = {"syrk": 1.0, "trsm": 1.0, "trmm": 1.0, "symm": 2.0, "syr2k": 2.0}   # x n^3
+
+
+def serve_requests(torch, count, seed=0xC0F7, nmin=128, nmax=8192):
+    """count requests: op by the mix, n log-uniform [nmin, nmax] (k = n), tier uniform
+    over plain / split / ozaki; operands U[-1,1] on the device (TRSM: diag n+U[0,1])."""
+    g = np.random.default_rng(seed)
+    ops = g.choice([o for o, _ in SERVE_MIX], size=count, p=[p for _, p in SERVE_MIX])
+    ns = np.rint(np.exp(g.uniform(np.log(nmin), np.log(nmax), count))).astype(np.int64)
+    tiers = g.choice(["plain", "split", "ozaki"], size=count)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    reqs, keep = [], []
+    for op, n, tier in zip(ops.tolist(), ns.tolist(), tiers.tolist()):
+        dt = torch.float64 if tier == "ozaki" else torch.float32
+
+        def rnd(n_):
+            return torch.rand(n_ * n_, device="cuda", dtype=dt, generator=gen) * 2 - 1
+        A, C = rnd(n), rnd(n)
+        B = rnd(n) if op in ("syr2k", "trmm", "symm") else None
+        r = dict(op=op, tier=tier, m=n, n=n, k=n, nb=256, alpha=1.0, beta=0.0, A=A.data_ptr(),
+                 lda=n, B=B.data_ptr() if B is not None else 0, ldb=n, C=C.data_ptr(), ldc=n,
+                 flops=SERVE_FLOPS[op] * float(n) ** 3)
+        C0 = None
+        if op == "trsm":
+            A.view(n, n).diagonal().copy_(n + torch.rand(n, device="cuda", dtype=dt, generator=gen))
+            C0 = C.clone()                # B is solved in place: restored before each pass
+        reqs.append(r)
+        keep.append((A, B, C, C0))
+    return reqs, keep
+
+
+def checksum(torch, x):
+    v = x.view(torch.int32 if x.dtype == torch.float32 else torch.int64).to(torch.int64)
+    w = torch.arange(1, v.numel() + 1, device=v.device, dtype=torch.int64) % 1000003
+    return (int(v.sum().item()), int((v * w).sum().item()))
+
+
+def pct(x, q):
+    return float(np.percentile(np.asarray(x), q))
+
+
+def run_serve(args, torch, tensorblas):
+    count = args.count or 500
+    nslots = args.slots
+    reqs, keep = serve_requests(torch, count)
+    total_flops = sum(r["flops"] for r in reqs)
+    log(f"bench: serve {count} requests, {total_flops / 1e12:.2f} TFLOP")
+
+    def restore():
+        for _, _, C, C0 in keep:
+            if C0 is not None:
+                C.copy_(C0)
+        torch.cuda.synchronize()
+    slots = [tensorblas.Handle(torch.cuda.Stream()) for _ in range(nslots)]
+    budget = int(args.budget_gib * 2 ** 30)
+    # warm-up on slot 0 sizes its workspace; every slot then reserves the same
+    for r in reqs:
+        tensorblas.serve(slots[:1], budget, [dict(r, arrival_us=0.0)])
+    ws = slots[0].workspace_sizes()
+    for h in slots[1:]:
+        h.reserve_workspace(*ws)
+    # isolated: each request alone on an idle GPU
+    restore()
+    iso = []
+    for r in reqs:
+        iso.append(tensorblas.serve(slots[:1], budget, [dict(r, arrival_us=0.0)])[0]["done_us"])
+    alone = [checksum(torch, C) for _, _, C, _ in keep]
+    # offline: everything available at t = 0, all slots, one lane (no admission limit
+    # beyond slots and budget): the best throughput the device reaches on this mix
+    restore()
+    off = tensorblas.serve(slots, budget, [dict(r, arrival_us=0.0) for r in reqs])
+    makespan = max(x["done_us"] for x in off)
+    cap = count / makespan                                      # requests per microsecond
+    # served: Poisson arrivals at args.load of the offline capacity, two admission lanes
+    restore()
+    g = np.random.default_rng(0xC0F7 + 1)
+    arr = np.cumsum(g.exponential(1.0 / (args.load * cap), count)) - 0.0
+    arr -= arr[0]
+    res = tensorblas.serve(slots, budget, [dict(r, arrival_us=float(a)) for r, a in zip(reqs, arr)],
+                           large_us=args.large_ms * 1e3, max_large=args.max_large)
+    lat = [x["done_us"] - a for x, a in zip(res, arr)]
+    ratio = [l_ / i_ for l_, i_ in zip(lat, iso)]
+    mix_sum = [checksum(torch, C) for _, _, C, _ in keep]
+    mismatch = sum(1 for a, b in zip(alone, mix_sum) if a != b)
+    span = max(x["done_us"] for x in res) - float(arr[0])
+    thr = total_flops / (span * 1e-6) / 1e12
+    thr_off = total_flops / (makespan * 1e-6) / 1e12
+    by_tier = {}
+    for t in ("plain", "split", "ozaki"):
+        li = [l_ for l_, r in zip(lat, reqs) if r["tier"] == t]
+        by_tier[t] = {"count": len(li), "p50_ms": round(pct(li, 50) / 1e3, 3),
+                      "p99_ms": round(pct(li, 99) / 1e3, 3)}
+    line = line_base("served throughput TFLOP/s with p50/p99 latency (Poisson arrivals)", thr,
+                     "TFLOP/s", 1, 1, span / 1e3, "per request tier: f16 plain / f32 split / f64 Ozaki",
+                     "synthetic: U[-1,1] operands resident in HBM (TRSM: diag n+U[0,1]), seeded",
+                     {"workload": f"{count} requests, mix SYRK 40/TRSM 30/TRMM 15/SYMM 10/SYR2K 5, "
+                                  f"n log-uniform [128, 8192], k=n, tier uniform over plain/split/ozaki",
+                      "slots": nslots, "hbm_budget_gib": args.budget_gib, "load": args.load,
+                      "large_ms": args.large_ms, "max_large": args.max_large,
+                      "total_flops": total_flops})
+    line.update({
+        "latency_ms": {"p50": round(pct(lat, 50) / 1e3, 3), "p99": round(pct(lat, 99) / 1e3, 3)},
+        "isolated_latency_ms": {"p50": round(pct(iso, 50) / 1e3, 3), "p99": round(pct(iso, 99) / 1e3, 3)},
+        "latency_over_isolated": {"p50": round(pct(ratio, 50), 3), "p99": round(pct(ratio, 99), 3)},
+        "p99_latency_vs_isolated_p99": round(pct(lat, 99) / pct(iso, 99), 3),
+        "offline": {"makespan_ms": round(makespan / 1e3, 3), "tflops": round(thr_off, 3)},
+        "throughput_vs_offline": round(thr / thr_off, 3),
+        "by_tier": by_tier,
+        "results_equal_alone": {"checked": count, "mismatches": mismatch},
+        "refused": sum(1 for x in res if x["status"] == 3),
+        "errors": sum(1 for x in res if x["status"] not in (0, 3)),
+    })
+    for h in slots:
+        h.close()
+    return line
+
+
 # ----------------------------------------------------------------- microbench
 def run_gemm(args, torch, tensorblas):
     M, N, K = args.m or 8192, args.n or 8192, args.k or 8192
@@ -546,7 +671,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="trsm",
-                    choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched"])
+                    choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched", "serve"])
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gib", type=float, default=32.0)
+    ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--large-ms", type=float, default=3.0)
+    ap.add_argument("--max-large", type=int, default=1)
     ap.add_argument("--count", type=int, default=0)
     ap.add_argument("--tier", default="")
     ap.add_argument("--n", type=int, default=0)
@@ -565,7 +695,7 @@ def main():
     import torch
     import tensorblas
     run = {"trsm": run_trsm, "syrk": run_syrk, "ozaki": run_ozaki, "gemm": run_gemm,
-           "copy": run_copy, "batched": run_batched}[args.workload]
+           "copy": run_copy, "batched": run_batched, "serve": run_serve}[args.workload]
     print(json.dumps(run(args, torch, tensorblas)), flush=True)
 
 

@@ -176,9 +176,12 @@ TB_API int tb_batch_destroy(tb_batch_t batch);
 /* ------------------------------------------------------------- request loop
  * tb_serve runs a stream of requests -- any routine, any tier, each with its own
  * arrival time -- on nslots handles (each with its own stream and workspace) within
- * an HBM budget.  A request is admitted first-come first-served as soon as a slot is
- * free and its bytes fit next to those already admitted; admitted requests run
- * concurrently on the device.  Operands are device pointers the caller keeps
+ * an HBM budget.  Requests go to one of two lanes by their estimated time alone
+ * (flops over a per-tier rate): those above cfg->large_us form the large lane, of
+ * which at most cfg->max_large run at once; the others never wait behind a large
+ * request.  Within a lane a request is admitted first-come first-served as soon as a
+ * slot is free and its bytes fit next to those already admitted; admitted requests
+ * run concurrently on the device.  Operands are device pointers the caller keeps
  * resident.  Times are device times (HIP events) in microseconds from the start of
  * the stream; the call returns when every request has finished.  A request whose
  * bytes exceed the budget is refused with TB_ERR_NOMEM.  Each request computes what
@@ -201,9 +204,15 @@ typedef struct {
 typedef struct {
     double admit_us, done_us;
     int slot, status;      /* status of the routine call (TB_ERR_NOMEM: refused) */
+    double est_us;         /* the estimate that chose the lane */
 } tb_request_result;
-TB_API int tb_serve(tb_handle_t* slots, int nslots, long hbm_budget, const tb_request* reqs, int count,
-             tb_request_result* results);
+typedef struct {
+    long hbm_budget;       /* bytes admitted requests may hold together */
+    double large_us;       /* estimated time alone above which a request is large (0: one lane) */
+    int max_large;         /* large requests running at once (>= 1) */
+} tb_serve_config;
+TB_API int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const tb_request* reqs,
+             int count, tb_request_result* results);
 /* Grow a handle's two device workspaces (routines; Ozaki products) to at least these
  * sizes, so that later calls never reallocate (which synchronizes the handle's stream);
  * tb_workspace_sizes reads the current sizes into out[0], out[1]. */

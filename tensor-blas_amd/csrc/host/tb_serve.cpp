@@ -4,9 +4,13 @@
 // completions are detected by polling events, and each request's times come from HIP
 // events against one start event, so they are device times.
 //
-// Admission is first-come first-served: the oldest waiting request is admitted as
-// soon as a slot is free and the bytes it holds fit the budget; later arrivals never
-// overtake it.  Work submitted on one slot stream never waits for another slot.
+// Admission: two first-come first-served lanes.  A request whose estimated time alone
+// (flops / per-tier rate, below) exceeds cfg->large_us is large; at most
+// cfg->max_large large requests run at once, so the longest requests do not share the
+// device with each other, and small requests never queue behind a large one.  Within
+// a lane the oldest waiting request is admitted as soon as a slot is free and the
+// bytes it holds fit the budget; later arrivals in its lane never overtake it.  Work
+// submitted on one slot stream never waits for another slot.
 #include <algorithm>
 #include <chrono>
 #include <thread>
@@ -40,6 +44,29 @@ double operand_elems(const tb_request& r)
 // bounded above by twice the operands.
 size_t default_bytes(const tb_request& r) { return (size_t)(2.0 * operand_elems(r) * elem_bytes(r.tier)); }
 
+// Algorithmic flops of a request and its estimated time alone: a fixed launch cost
+// plus flops at a per-tier rate (MI355X, large sizes; small sizes run below these
+// rates, which the fixed term covers in part).
+double req_flops(const tb_request& r)
+{
+    const double m = (double)r.m, n = (double)r.n, k = (double)r.k;
+    switch (r.op) {
+    case TB_REQ_SYRK: return n * n * k;
+    case TB_REQ_SYR2K: return 2 * n * n * k;
+    case TB_REQ_TRMM: return m * m * n;
+    case TB_REQ_TRSM: return m * n * n;
+    case TB_REQ_SYMM: return 2 * m * m * n;
+    default: return 0;
+    }
+}
+
+double est_us(const tb_request& r)
+{
+    static const double tflops[5] = {250.0, 150.0, 28.0, 70.0, 40.0};   // by tier
+    const int t = r.tier >= 0 && r.tier < 5 ? r.tier : 0;
+    return 50.0 + req_flops(r) / (tflops[t] * 1e6);
+}
+
 int launch(tb_handle_t h, const tb_request& r)
 {
     switch (r.op) {
@@ -62,6 +89,7 @@ struct Running {
     int req;
     int slot;
     size_t bytes;
+    bool large;
 };
 
 }  // namespace
@@ -83,11 +111,13 @@ int tb_workspace_sizes(tb_handle_t h, long* out)
     return TB_SUCCESS;
 }
 
-int tb_serve(tb_handle_t* slots, int nslots, long hbm_budget, const tb_request* reqs, int count,
-             tb_request_result* out)
+int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const tb_request* reqs,
+             int count, tb_request_result* out)
 {
-    if (!slots || nslots <= 0 || hbm_budget <= 0 || count < 0 || (count > 0 && (!reqs || !out)))
+    if (!slots || nslots <= 0 || !cfg || cfg->hbm_budget <= 0 || cfg->large_us < 0 ||
+        cfg->max_large < 1 || count < 0 || (count > 0 && (!reqs || !out)))
         return TB_ERR_INVALID;
+    const size_t budget = (size_t)cfg->hbm_budget;
     for (int s = 0; s < nslots; ++s)
         if (!slots[s]) return TB_ERR_INVALID;
     for (int i = 0; i < count; ++i) {
@@ -96,15 +126,20 @@ int tb_serve(tb_handle_t* slots, int nslots, long hbm_budget, const tb_request* 
         out[i].admit_us = out[i].done_us = -1.0;
         out[i].slot = -1;
         out[i].status = TB_SUCCESS;
+        out[i].est_us = est_us(reqs[i]);
     }
     if (count == 0) return TB_SUCCESS;
 
-    std::vector<int> order(count);
-    for (int i = 0; i < count; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return reqs[a].arrival_us < reqs[b].arrival_us; });
+    // the two lanes, each in arrival order
+    std::vector<int> lane[2];
     std::vector<size_t> need(count);
-    for (int i = 0; i < count; ++i) need[i] = reqs[i].bytes > 0 ? (size_t)reqs[i].bytes : default_bytes(reqs[i]);
+    for (int i = 0; i < count; ++i) {
+        need[i] = reqs[i].bytes > 0 ? (size_t)reqs[i].bytes : default_bytes(reqs[i]);
+        lane[cfg->large_us > 0 && out[i].est_us > cfg->large_us ? 1 : 0].push_back(i);
+    }
+    for (auto& l : lane)
+        std::stable_sort(l.begin(), l.end(),
+                         [&](int a, int b) { return reqs[a].arrival_us < reqs[b].arrival_us; });
 
     // one completion event per slot, one start event
     std::vector<hipEvent_t> done(nslots, nullptr);
@@ -125,7 +160,8 @@ int tb_serve(tb_handle_t* slots, int nslots, long hbm_budget, const tb_request* 
     for (int s = nslots - 1; s >= 0; --s) idle.push_back(s);
     std::vector<Running> run;
     size_t used = 0;
-    int next = 0, finished = 0;
+    int large_running = 0, finished = 0;
+    size_t next[2] = {0, 0};
     while (st == TB_SUCCESS && finished < count) {
         bool progress = false;
         // completions
@@ -142,38 +178,42 @@ int tb_serve(tb_handle_t* slots, int nslots, long hbm_budget, const tb_request* 
             }
             out[run[q].req].done_us = 1e3 * (double)ms;
             used -= run[q].bytes;
+            large_running -= run[q].large ? 1 : 0;
             idle.push_back(run[q].slot);
             run[q] = run.back();
             run.pop_back();
             ++finished;
             progress = true;
         }
-        // admissions, oldest first
+        // admissions, oldest first within each lane; the small lane first
         const double now_us =
             std::chrono::duration<double, std::micro>(Clock::now() - host0).count();
-        while (st == TB_SUCCESS && next < count) {
-            const int r = order[next];
-            if (reqs[r].arrival_us > now_us) break;
-            if (need[r] > (size_t)hbm_budget) {            // can never fit: refuse it
-                out[r].status = TB_ERR_NOMEM;
+        for (int L = 0; L < 2 && st == TB_SUCCESS; ++L) {
+            while (next[L] < lane[L].size()) {
+                const int r = lane[L][next[L]];
+                if (reqs[r].arrival_us > now_us) break;
+                if (need[r] > budget) {                       // can never fit: refuse it
+                    out[r].status = TB_ERR_NOMEM;
+                    out[r].admit_us = now_us;
+                    ++next[L];
+                    ++finished;
+                    progress = true;
+                    continue;
+                }
+                if (idle.empty() || used + need[r] > budget) break;
+                if (L == 1 && large_running >= cfg->max_large) break;
+                const int s = idle.back();
+                idle.pop_back();
                 out[r].admit_us = now_us;
-                ++next;
-                ++finished;
+                out[r].slot = s;
+                out[r].status = launch(slots[s], reqs[r]);
+                if (hipEventRecord(done[s], slots[s]->stream) != hipSuccess) st = TB_ERR_DEVICE;
+                used += need[r];
+                large_running += L;
+                run.push_back({r, s, need[r], L == 1});
+                ++next[L];
                 progress = true;
-                continue;
             }
-            if (idle.empty() || used + need[r] > (size_t)hbm_budget) break;
-            const int s = idle.back();
-            idle.pop_back();
-            out[r].admit_us = now_us;
-            out[r].slot = s;
-            const int rs = launch(slots[s], reqs[r]);
-            if (hipEventRecord(done[s], slots[s]->stream) != hipSuccess) st = TB_ERR_DEVICE;
-            out[r].status = rs;
-            used += need[r];
-            run.push_back({r, s, need[r]});
-            ++next;
-            progress = true;
         }
         if (!progress) std::this_thread::yield();
     }

@@ -64,7 +64,7 @@ def lib():
             "tb_batch_run": [_P],
             "tb_batch_info": [_P, _P, _I],
             "tb_batch_destroy": [_P],
-            "tb_serve": [_P, _I, _L, _P, _I, _P],
+            "tb_serve": [_P, _I, _P, _P, _I, _P],
             "tb_reserve_workspace": [_P, _L, _L],
             "tb_workspace_sizes": [_P, _P],
         }
@@ -256,16 +256,23 @@ class Request(ctypes.Structure):
 
 class RequestResult(ctypes.Structure):
     _fields_ = [("admit_us", ctypes.c_double), ("done_us", ctypes.c_double),
-                ("slot", ctypes.c_int), ("status", ctypes.c_int)]
+                ("slot", ctypes.c_int), ("status", ctypes.c_int), ("est_us", ctypes.c_double)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_long), ("large_us", ctypes.c_double),
+                ("max_large", ctypes.c_int)]
 
 
 REQ_OPS = {"syrk": 0, "syr2k": 1, "trmm": 2, "trsm": 3, "symm": 4}
 
 
-def serve(slots, budget_bytes, requests):
+def serve(slots, budget_bytes, requests, large_us=0.0, max_large=1):
     """Run a request stream (tb_serve) on the given handles; requests are dicts with
     op, tier, m, n, k, nb, alpha, beta, A, lda, B, ldb, C, ldc, arrival_us, bytes.
-    Returns a list of dicts admit_us, done_us, slot, status."""
+    large_us > 0 splits admission into a small and a large lane (at most max_large
+    large requests at once).  Returns a list of dicts admit_us, done_us, slot, status,
+    est_us."""
     n = len(requests)
     arr = (Request * max(1, n))()
     for i, r in enumerate(requests):
@@ -275,9 +282,10 @@ def serve(slots, budget_bytes, requests):
                          float(r.get("arrival_us", 0.0)), int(r.get("bytes", 0)))
     res = (RequestResult * max(1, n))()
     hs = (ctypes.c_void_p * len(slots))(*[s.ptr.value for s in slots])
-    _check("tb_serve", lib().tb_serve(hs, len(slots), int(budget_bytes), arr, n, res))
+    cfg = ServeConfig(int(budget_bytes), float(large_us), int(max_large))
+    _check("tb_serve", lib().tb_serve(hs, len(slots), ctypes.byref(cfg), arr, n, res))
     return [dict(admit_us=res[i].admit_us, done_us=res[i].done_us, slot=res[i].slot,
-                 status=res[i].status) for i in range(n)]
+                 status=res[i].status, est_us=res[i].est_us) for i in range(n)]
 
 
 # --------------------------------------------------------- host-only planners

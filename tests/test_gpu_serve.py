@@ -111,3 +111,26 @@ def test_serve_rejects_bad_arguments(tbm):
     with pytest.raises(tbm.TensorBLASError):
         tbm.serve(hs, 1 << 20, [dict(op="syrk", tier="plain", n=1, k=1, A=z.data_ptr(), lda=1,
                                      C=z.data_ptr(), ldc=1, arrival_us=-1.0)])
+
+
+def test_serve_lanes(tbm):
+    """Requests estimated above large_us run at most max_large at a time; small ones
+    keep flowing; every result still equals the isolated one."""
+    reqs, keep = make_requests(9, sizes=(128, 448))
+    alone, est = [], []
+    h1 = slots(tbm, 1)
+    for r, (_, _, C, _) in zip(reqs, keep):
+        est.append(tbm.serve(h1, 1 << 34, [dict(r, arrival_us=0.0)])[0]["est_us"])
+        alone.append(C.clone())
+    restore(keep)
+    thr = float(np.median(est))                # the library's own estimates split the set
+    hs = slots(tbm, 6)
+    res = tbm.serve(hs, 1 << 34, [dict(r, arrival_us=0.0) for r in reqs], large_us=thr, max_large=1)
+    large = [x for x in res if x["est_us"] > thr]
+    assert 0 < len(large) < len(res)
+    large.sort(key=lambda x: x["admit_us"])
+    for a, b in zip(large, large[1:]):
+        assert b["admit_us"] + 200.0 >= a["done_us"]          # never two large at once
+    for i, x in enumerate(res):
+        assert x["status"] == 0
+        assert torch.equal(keep[i][2], alone[i]), (reqs[i]["op"], reqs[i]["tier"], reqs[i]["n"])
