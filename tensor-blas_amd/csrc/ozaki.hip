@@ -86,90 +86,164 @@ ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int row
     }
 }
 
-constexpr int OBM = 64, OBN = 64, OBK = 64, OPAD = 16;
-constexpr int OLDK = OBK + OPAD;   // bytes per LDS row
+constexpr int OBM = 64, OBN = 64, OBK = 64;   // tile and k block (64 int8 = one LDS row)
 constexpr int kMinSlices = 3, kMaxSlices = 10;
+constexpr int kOzStages = 2;
 
-// One 64 x 64 tile of C per workgroup; each wave owns a 32 x 32 block.
+// direct-to-LDS load of 16 bytes per lane (see gemm16_kernel.hpp lds_dma16)
+__device__ __forceinline__ void oz_dma16(const void* g, unsigned lds_base)
+{
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds_base), "v"(g)
+                 : "memory", "m0");
+}
+#define TB_OZ_WAIT_BARRIER() asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory")
+
+// Lower-triangular tile g of a T x T tile grid, walked in bands of kBand tile rows:
+// inside a band column by column (rows fastest), first the columns left of the band,
+// then its triangle.  Consecutive tiles -- those an XCD runs together -- then share
+// kBand row panels and a few column panels instead of one row panel and many columns.
+constexpr int kBand = 8;
+
+__device__ __forceinline__ void lower_tile_banded(int g, int T, int& ti, int& tj)
+{
+    int r0 = 0;
+    for (;;) {
+        const int gb = min(kBand, T - r0);                        // rows of this band
+        const int cnt = r0 * gb + gb * (gb + 1) / 2;
+        if (g < cnt || r0 + gb >= T) break;
+        g -= cnt;
+        r0 += gb;
+    }
+    const int gb = min(kBand, T - r0);
+    if (g < r0 * gb) {                                            // columns left of the band
+        tj = g / gb;
+        ti = r0 + g % gb;
+        return;
+    }
+    g -= r0 * gb;
+    tj = r0;                                                      // triangle: column c has gb - c rows
+    for (int c = 0; c < gb; ++c) {
+        const int rows = gb - c;
+        if (g < rows) {
+            tj = r0 + c;
+            ti = r0 + c + g;
+            return;
+        }
+        g -= rows;
+    }
+    ti = tj = T - 1;                                              // unreachable for g < total
+}
+
+// One 64 x 64 tile of C per workgroup; each wave owns a 32 x 32 block.  A k block
+// holds the 64-byte rows of all S slices of both panels (S * 8 KB) in LDS, the four
+// 16-byte units of row r at unit ^ ((r >> 2) & 3); kOzStages blocks are in flight,
+// block kb+1's loads overlapping block kb's MFMAs.
 template <int S>
 __global__ void __launch_bounds__(256)
 ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
 {
-    __shared__ __attribute__((aligned(16))) int8_t sA[S * OBM * OLDK];   // [S][OBM][OLDK]
-    __shared__ __attribute__((aligned(16))) int8_t sB[S * OBN * OLDK];
+    constexpr int PLANE = OBM * OBK;                 // bytes of one slice of one panel
+    constexpr int STAGE = 2 * S * PLANE;             // A slices, then B slices
+    static_assert(kOzStages * STAGE <= 160 * 1024, "LDS");
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kOzStages * STAGE];
 
     const int g = xcd_remap(blockIdx.x, totalTiles);
     int ti, tj;
     if (P.lowerC) {
-        ti = (int)((__builtin_sqrtf(8.0f * (float)g + 1.0f) - 1.0f) * 0.5f);
-        while ((ti + 1) * (ti + 2) / 2 <= g) ++ti;
-        while (ti * (ti + 1) / 2 > g) --ti;
-        tj = g - ti * (ti + 1) / 2;
+        lower_tile_banded(g, tilesM, ti, tj);
     } else {
         ti = g % tilesM;
         tj = g / tilesM;
     }
     const int i0 = ti * OBM, j0 = tj * OBN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
+
+    // this wave's direct-to-LDS loads: instruction q moves 16 rows (1 KB) of one slice
+    // of one panel; 8*S instructions per block, 2*S per wave
+    constexpr int NQ = 2 * S;
+    const int8_t* gsrc[NQ];
+    unsigned ldo[NQ];
+    int pair_src = -1;
+    const unsigned smem_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)smem;
+    auto dma_setup = [&](int pair) {
+        const int8_t* GA = pair ? P.sa2 : P.sa;
+        const int8_t* GB = pair ? P.sb2 : P.sb;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int inst = wave + q * 4;                  // 0 .. 8S-1
+            const int panel = inst / (4 * S);               // 0: A, 1: B
+            const int rest = inst % (4 * S);
+            const int sl = rest / 4, blk = rest % 4;
+            const int pos = blk * 64 + lane;                // physical unit in the slice
+            const int rl = pos >> 2;
+            const int u = (pos & 3) ^ ((rl >> 2) & 3);      // logical unit stored there
+            const int rows = panel ? P.n : P.m;
+            int r = (panel ? j0 : i0) + rl;
+            r = r < rows ? r : rows - 1;
+            gsrc[q] = (panel ? GB : GA) + ((long long)sl * rows + r) * P.kpad + u * 16;
+            ldo[q] = (unsigned)((panel * S + sl) * PLANE + blk * 1024);
+        }
+    };
+    const int npairs = P.sa2 ? 2 : 1;
+    const int kblocks = P.kpad / OBK;
+    const int nblk = npairs * kblocks;
+    auto issue = [&](int b) {
+        const int pair = b >= kblocks;
+        if (pair != pair_src) {
+            dma_setup(pair);
+            pair_src = pair;
+        }
+        const int kb = b - pair * kblocks;
+        const unsigned base = smem_lds + (unsigned)((b % kOzStages) * STAGE);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) oz_dma16(gsrc[q] + kb * OBK, base + ldo[q]);
+    };
 
     intx16_t acc[S];
     double facc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) facc[r] = 0.0;
+#pragma unroll
+    for (int q = 0; q < S; ++q) acc[q] = intx16_t{};
 
-    const int npairs = P.sa2 ? 2 : 1;
-    const int kblocks = P.kpad / OBK;
-    const int cblocks = P.kchunk / OBK;       // k blocks per exactness chunk
-    const int lrow = tid >> 2, lseg = (tid & 3) * 16;
-
-    for (int pair = 0; pair < npairs; ++pair) {
-        const int8_t* GA = pair ? P.sa2 : P.sa;
-        const int8_t* GB = pair ? P.sb2 : P.sb;
-        for (int c0 = 0; c0 < kblocks; c0 += cblocks) {
-            const int c1 = min(kblocks, c0 + cblocks);
+    const int cblocks = P.kchunk / OBK;          // k blocks per exactness chunk
+    const int fr = lane & 31, fsw = (fr >> 2) & 3, fu = lane >> 5;
+    issue(0);
+    for (int b = 0; b < nblk; ++b) {
+        TB_OZ_WAIT_BARRIER();                    // block b landed; block b-1's slot is free
+        if (b + 1 < nblk) issue(b + 1);
+        const int8_t* sA = smem + (b % kOzStages) * STAGE;
+        const int8_t* sB = sA + S * PLANE;
 #pragma unroll
-            for (int q = 0; q < S; ++q) acc[q] = intx16_t{};
-            for (int kb = c0; kb < c1; ++kb) {
-                __syncthreads();
-                // stage all slices of both panels for this k block
+        for (int kk = 0; kk < 2; ++kk) {         // two 32-k steps = units (2kk, 2kk+1)
+            const int uo = ((2 * kk + fu) ^ fsw) * 16;
+            intx4_t af[S], bf[S];
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int ra = i0 + lrow, rb = j0 + lrow;
-                    intx4_t va = intx4_t{}, vb = intx4_t{};
-                    if (ra < P.m)
-                        va = *reinterpret_cast<const intx4_t*>(
-                            GA + ((long long)s * P.m + ra) * P.kpad + kb * OBK + lseg);
-                    if (rb < P.n)
-                        vb = *reinterpret_cast<const intx4_t*>(
-                            GB + ((long long)s * P.n + rb) * P.kpad + kb * OBK + lseg);
-                    *reinterpret_cast<intx4_t*>(sA + (s * OBM + lrow) * OLDK + lseg) = va;
-                    *reinterpret_cast<intx4_t*>(sB + (s * OBN + lrow) * OLDK + lseg) = vb;
-                }
-                __syncthreads();
-                const int fr = lane & 31, fk = (lane >> 5) * 16;
-#pragma unroll
-                for (int kk = 0; kk < OBK; kk += 32) {
-                    intx4_t af[S], bf[S];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        af[s] = *reinterpret_cast<const intx4_t*>(sA + (s * OBM + wm * 32 + fr) * OLDK + kk + fk);
-                        bf[s] = *reinterpret_cast<const intx4_t*>(sB + (s * OBN + wn * 32 + fr) * OLDK + kk + fk);
-                    }
-#pragma unroll
-                    for (int gg = 0; gg < S; ++gg) {
-#pragma unroll
-                        for (int s = 0; s <= gg; ++s)
-                            acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
-                    }
-                }
+            for (int s = 0; s < S; ++s) {
+                af[s] = *reinterpret_cast<const intx4_t*>(sA + s * PLANE + (wm * 32 + fr) * OBK + uo);
+                bf[s] = *reinterpret_cast<const intx4_t*>(sB + s * PLANE + (wn * 32 + fr) * OBK + uo);
             }
-            // chunk done: D_g into FP64, increasing g
+            // D_g += A_s B_{g-s}: consecutive MFMAs update different D_g
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+#pragma unroll
+                for (int gg = s; gg < S; ++gg)
+                    acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
+            }
+        }
+        const int kb = b % kblocks;
+        if ((kb + 1) % cblocks == 0 || kb + 1 == kblocks) {
+            // exactness chunk done: D_g into FP64, increasing g
 #pragma unroll
             for (int gg = 0; gg < S; ++gg) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
                     facc[r] = facc[r] + __builtin_ldexp((double)acc[gg][r], -7 * (gg + 2));
+                acc[gg] = intx16_t{};
             }
         }
     }
