@@ -154,8 +154,14 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
     if (P.lowerC) {
         lower_tile_banded(g, tilesM, ti, tj);
     } else {
-        ti = g % tilesM;
-        tj = g / tilesM;
+        // groups of kBand tile rows, column by column inside a group
+        const int tilesN = totalTiles / tilesM;
+        const int per = kBand * tilesN;
+        const int first = (g / per) * kBand;
+        const int gsz = min(tilesM - first, kBand);
+        const int in = g - (g / per) * per;
+        ti = first + in % gsz;
+        tj = in / gsz;
     }
     const int i0 = ti * OBM, j0 = tj * OBN;
     const int tid = threadIdx.x, lane = tid & 63;
