@@ -101,4 +101,69 @@ __device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long lon
     }
 }
 
+// 64 rows x 64 plane columns per block through an LDS tile, so both the source read
+// (rows or k contiguous) and the 16-byte plane writes are coalesced.  e_used (split
+// tier, optional) receives the row exponents the planes were scaled with, for a GEMM
+// that runs while e itself keeps growing.
+// Block (bx, by) of the conversion: rows by*64.., plane columns bx*64.. (t: a 64 x 65
+// float LDS tile).  e_used is written by the blocks with bx == 0.
+template <int NPL>
+__device__ __forceinline__ void split_planes_block(const float* __restrict__ X, long long ld, int mode,
+                                                   int rows, int K, const int* __restrict__ e,
+                                                   int* __restrict__ e_used,
+                                                   _Float16* __restrict__ out, long long ldp,
+                                                   long long sp, int kshift, int bx, int by,
+                                                   float (*t)[65])
+{
+    const int r0 = by * 64, c0 = bx * 64;   // c: plane column = kshift + k
+    const int tid = threadIdx.x;
+    for (int q = tid; q < 64 * 64; q += 256) {
+        int rl, cl;
+        if (mode == kOpT) { cl = q % 64; rl = q / 64; }
+        else { rl = q % 64; cl = q / 64; }
+        const int r = r0 + rl, k = c0 + cl - kshift;
+        float v = 0.0f;
+        if (r < rows && k >= 0 && k < K) {
+            if (mode == kOpN) v = X[(long long)r + (long long)k * ld];
+            else if (mode == kOpT) v = X[(long long)k + (long long)r * ld];
+            else if (r >= k) v = X[(long long)r + (long long)k * ld];
+            else if (mode == kOpSymL) v = X[(long long)k + (long long)r * ld];
+        }
+        t[rl][cl] = v;
+    }
+    __syncthreads();
+    const int rl = tid >> 2, cl0 = (tid & 3) * 16;
+    const int r = r0 + rl;
+    if (r >= rows) return;
+    int sc = 0;
+    if (NPL >= 2) {
+        const int ex = e[r];
+        sc = 15 - (ex < -1000 ? 0 : (ex > 1000 ? 1000 : ex));
+        if (e_used && bx == 0 && (tid & 3) == 0) e_used[r] = ex;
+    }
+    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    for (int half = 0; half < 2; ++half) {
+        half8 hh, mm, ll;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float x = t[rl][cl0 + half * 8 + j];
+            if (NPL == 1) {
+                hh[j] = (_Float16)x;
+            } else {
+                const float xs = __builtin_ldexpf(x, sc);
+                const _Float16 h = (_Float16)xs;
+                const float r1 = (xs - (float)h) * 2048.0f;
+                const _Float16 m = (_Float16)r1;
+                hh[j] = h;
+                mm[j] = m;
+                if (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
+            }
+        }
+        _Float16* dst = out + (long long)r * ldp + c0 + cl0 + half * 8;
+        *reinterpret_cast<half8*>(dst) = hh;
+        if (NPL >= 2) *reinterpret_cast<half8*>(dst + sp) = mm;
+        if (NPL == 3) *reinterpret_cast<half8*>(dst + 2 * sp) = ll;
+    }
+}
+
 }  // namespace tb

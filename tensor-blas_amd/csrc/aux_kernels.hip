@@ -128,10 +128,6 @@ __global__ void add_matrix_f32_kernel(long long m, long long n, float* x, long l
             x[i + j * ldx] += y[i + j * ldy];
 }
 
-// 64 rows x 64 plane columns per block through an LDS tile, so both the source read
-// (rows or k contiguous) and the 16-byte plane writes are coalesced.  e_used (split
-// tier, optional) receives the row exponents the planes were scaled with, for a GEMM
-// that runs while e itself keeps growing.
 template <int NPL>
 __global__ void __launch_bounds__(256)
 split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int rows, int K,
@@ -139,55 +135,8 @@ split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int row
                     _Float16* __restrict__ out, long long ldp, long long sp, int kshift)
 {
     __shared__ float t[64][65];
-    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;   // c: plane column = kshift + k
-    const int tid = threadIdx.x;
-    for (int q = tid; q < 64 * 64; q += 256) {
-        int rl, cl;
-        if (mode == kOpT) { cl = q % 64; rl = q / 64; }
-        else { rl = q % 64; cl = q / 64; }
-        const int r = r0 + rl, k = c0 + cl - kshift;
-        float v = 0.0f;
-        if (r < rows && k >= 0 && k < K) {
-            if (mode == kOpN) v = X[(long long)r + (long long)k * ld];
-            else if (mode == kOpT) v = X[(long long)k + (long long)r * ld];
-            else if (r >= k) v = X[(long long)r + (long long)k * ld];
-            else if (mode == kOpSymL) v = X[(long long)k + (long long)r * ld];
-        }
-        t[rl][cl] = v;
-    }
-    __syncthreads();
-    const int rl = tid >> 2, cl0 = (tid & 3) * 16;
-    const int r = r0 + rl;
-    if (r >= rows) return;
-    int sc = 0;
-    if (NPL >= 2) {
-        const int ex = e[r];
-        sc = 15 - (ex < -1000 ? 0 : (ex > 1000 ? 1000 : ex));
-        if (e_used && blockIdx.x == 0 && (tid & 3) == 0) e_used[r] = ex;
-    }
-    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-    for (int half = 0; half < 2; ++half) {
-        half8 hh, mm, ll;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float x = t[rl][cl0 + half * 8 + j];
-            if (NPL == 1) {
-                hh[j] = (_Float16)x;
-            } else {
-                const float xs = __builtin_ldexpf(x, sc);
-                const _Float16 h = (_Float16)xs;
-                const float r1 = (xs - (float)h) * 2048.0f;
-                const _Float16 m = (_Float16)r1;
-                hh[j] = h;
-                mm[j] = m;
-                if (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
-            }
-        }
-        _Float16* dst = out + (long long)r * ldp + c0 + cl0 + half * 8;
-        *reinterpret_cast<half8*>(dst) = hh;
-        if (NPL >= 2) *reinterpret_cast<half8*>(dst + sp) = mm;
-        if (NPL == 3) *reinterpret_cast<half8*>(dst + 2 * sp) = ll;
-    }
+    split_planes_block<NPL>(X, ld, mode, rows, K, e, e_used, out, ldp, sp, kshift, blockIdx.x,
+                            blockIdx.y, t);
 }
 
 dim3 grid2d(long long m, long long n)

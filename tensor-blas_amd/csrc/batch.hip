@@ -2,11 +2,13 @@
 //
 // One workgroup per CU loops: take the next work item from an atomic counter (items
 // are sorted by decreasing work), run it, until the list is exhausted.  A TRSM item
-// walks its instance's recursion for 64 rows: each leaf X = (B + B W^T) D^-1 is one
-// 64 x 256 tile, each update B2 -= X1 L21^T is ceil(w / 256) such tiles.  Every tile
-// converts its FP32 operands while staging them (the plane values the single-call
-// path pre-converts, with the same exponents) and uses the same MFMA sequence per
-// output element, so a batched instance equals its single call bit for bit.
+// walks its instance's recursion for 64 rows: at each step it converts its rows of the
+// solved block (the leaf's B block, or the update's X1) into FP16 planes in its own
+// scratch, with the exponents the single call uses; then each leaf X = (B + B W^T) D^-1
+// is one 64 x 256 tile and each update B2 -= X1 L21^T is ceil(w / 256) tiles, reading
+// L and W^T from planes the prologue made.  All these tiles stage their operands with
+// direct-to-LDS loads.  The plane values and the MFMA sequence per output element are
+// the single call's, so a batched instance equals its single call bit for bit.
 #include <hip/hip_runtime.h>
 
 #include "aux_device.hpp"
@@ -18,24 +20,33 @@ namespace tb {
 
 namespace {
 
-// tile configurations: TRSM panel tiles and the 64 x 256 SYRK tiles; 128 x 128 SYRK tiles
+constexpr int kPP = kSrcPlanes, kFF = kSrcF32;
+
+// stages of the plane-sourced panel / square tiles: as many as fit in the LDS
 template <int NPL>
-using CfgPanel = Gemm16Cfg<NPL, kSrcF32, kSrcF32, kPanelRows, kPanelCols, 1, 4, 2>;
+constexpr int panel_stages() { return NPL == 3 ? 2 : 3; }
 template <int NPL>
-using CfgSq = Gemm16Cfg<NPL, kSrcF32, kSrcF32, 128, 128, 2, 2, 2>;
+constexpr int sq_stages() { return NPL == 3 ? 3 : 4; }
+template <int NPL, int SRC>
+constexpr int panel_nst() { return SRC == kSrcPlanes ? panel_stages<NPL>() : 2; }
+
+template <int NPL, int SRC>
+using CfgPanel = Gemm16Cfg<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, 4, panel_nst<NPL, SRC>()>;
 template <int NPL>
-using LdsPanel = Gemm16Lds<CfgPanel<NPL>, kPanelRows, kPanelCols>;
+using CfgSq = Gemm16Cfg<NPL, kSrcPlanes, kSrcPlanes, 128, 128, 2, 2, sq_stages<NPL>()>;
+template <int NPL, int SRC>
+using LdsPanel = Gemm16Lds<CfgPanel<NPL, SRC>, kPanelRows, kPanelCols>;
 template <int NPL>
 using LdsSq = Gemm16Lds<CfgSq<NPL>, 128, 128>;
 
 constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 
-template <int NPL>
+template <int NPL, int SRC>
 __device__ __forceinline__ void panel_tile(const GemmProb& P, int i0, int j0, char* raw)
 {
-    auto* L = reinterpret_cast<LdsPanel<NPL>*>(raw);
-    gemm16_tile<NPL, kSrcF32, kSrcF32, kPanelRows, kPanelCols, 1, 4, 2>(P, 0, i0, j0, L->ring,
-                                                                        L->scA, L->scB, L->emax);
+    auto* L = reinterpret_cast<LdsPanel<NPL, SRC>*>(raw);
+    gemm16_tile<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, 4, panel_nst<NPL, SRC>()>(
+        P, 0, i0, j0, L->ring, L->scA, L->scB, L->emax);
 }
 
 // running exponents are raised by atomics at the L2: read them past the L1
@@ -44,63 +55,142 @@ __device__ __forceinline__ int load_exp(const int* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Rows i0 .. i0+63 (those < m) of the column-major X (ld), columns 0 .. K-1, into NPL
+// planes at dst (plane p, row r, column k at dst[p*sp + r*ldp + k]) with columns
+// K .. round_up(K, 64) and rows >= m zero.  Split planes are scaled per row by
+// 2^(15 - e[row]) exactly as split_planes_block converts them.  t: a 64 x 65 LDS tile.
+template <int NPL>
+__device__ __forceinline__ void panel_planes(const float* __restrict__ X, long long ld, int m, int i0, int K,
+                             const int* __restrict__ e, _Float16* __restrict__ dst, long long ldp,
+                             long long sp, float (*t)[65])
+{
+    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    const int tid = threadIdx.x;
+    const int rl = tid >> 2, cl0 = (tid & 3) * 16;
+    int sc = 0;
+    if (NPL >= 2 && i0 + rl < m) {
+        const int ex = e[i0 + rl];
+        sc = 15 - (ex < -1000 ? 0 : (ex > 1000 ? 1000 : ex));
+    }
+    const int kend = (K + 63) / 64 * 64;
+    for (int c0 = 0; c0 < kend; c0 += 64) {
+        __syncthreads();                                 // t is free
+        for (int q = tid; q < 64 * 64; q += 256) {
+            const int rr = q & 63, cc = q >> 6;
+            const int r = i0 + rr, k = c0 + cc;
+            t[rr][cc] = (r < m && k < K) ? X[(long long)r + (long long)k * ld] : 0.0f;
+        }
+        __syncthreads();
+        for (int half = 0; half < 2; ++half) {
+            half8 hh, mm, ll;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float x = t[rl][cl0 + half * 8 + j];
+                if constexpr (NPL == 1) {
+                    hh[j] = (_Float16)x;
+                } else {
+                    const float xs = __builtin_ldexpf(x, sc);
+                    const _Float16 h = (_Float16)xs;
+                    const float r1 = (xs - (float)h) * 2048.0f;
+                    const _Float16 mq = (_Float16)r1;
+                    hh[j] = h;
+                    mm[j] = mq;
+                    if constexpr (NPL == 3) ll[j] = (_Float16)((r1 - (float)mq) * 2048.0f);
+                }
+            }
+            _Float16* d = dst + (long long)rl * ldp + c0 + cl0 + half * 8;
+            *reinterpret_cast<half8*>(d) = hh;
+            if constexpr (NPL >= 2) *reinterpret_cast<half8*>(d + sp) = mm;
+            if constexpr (NPL == 3) *reinterpret_cast<half8*>(d + 2 * sp) = ll;
+        }
+    }
+    // the tiles read these planes with direct-to-LDS loads: stores complete, then the
+    // workgroup's vector L1 drops whatever it held of the scratch
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // rows i0 .. i0+63 of one TRSM instance through its whole schedule
 template <int UNPL, int ANPL>
-__device__ void trsm_panel(const BatchInst& I, const BatchStep* __restrict__ steps, int i0,
-                           char* raw)
+__device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* __restrict__ steps, int i0,
+                           const BatchScratch& S, char* raw)
 {
     const int rows = min(kPanelRows, I.m - i0);
+    _Float16* scr = reinterpret_cast<_Float16*>(static_cast<char*>(S.base) +
+                                                (long long)blockIdx.x * S.bytes_per_wg);
+    // the scratch holds rows i0..i0+63: row i of the operand sits at scr + (i - i0)*ldp
+    const _Float16* scrA = scr - (long long)i0 * S.ldp;
+    auto* t = reinterpret_cast<float (*)[65]>(raw);
     for (int q = 0; q < I.nsteps; ++q) {
         const BatchStep st = steps[I.step0 + q];
         // the exponents this step's conversions use: the single-call path converts the
         // solved block once per step, before the step's epilogues raise eRow
+        __syncthreads();
         for (int r = threadIdx.x; r < rows; r += blockDim.x) I.eSnap[i0 + r] = load_exp(I.eRow + i0 + r);
         __syncthreads();
         GemmProb P = {};
         P.m = I.m;
         P.batch = 1;
+        P.modeA = P.modeB = kOpN;
         P.modeC = kCFull;
-        P.srcA = P.srcB = kSrcF32;
         P.eA = I.eSnap;
         P.eOut = I.eRow;
+        P.ldc = I.ldc;
+        P.beta = 1.0;
         if (st.type == 0) {
             // leaf: B(:, off:off+s) := (B + B W^T) D^-1 in place (one tile spans all s columns)
             const int off = st.a, s = st.b;
+            panel_planes<ANPL>(I.C + (long long)off * I.ldc, I.ldc, I.m, i0, s, I.eSnap, scr, S.ldp,
+                               S.sp, t);
             P.n = s;
             P.k = s;
-            P.A = I.C + (long long)off * I.ldc;
-            P.lda = I.ldc;
-            P.modeA = kOpN;
-            P.B = I.Z + st.zoff;
-            P.ldb = s;
-            P.modeB = kOpN;
+            P.A = scrA;
+            P.lda = S.ldp;
+            P.spA = S.sp;
+            P.B = static_cast<const _Float16*>(I.Zp) + (long long)st.eoff * kZpLd;
+            P.ldb = kZpLd;
+            P.spB = I.spZ;
+            P.srcA = P.srcB = kPP;
             P.C = I.C + (long long)off * I.ldc;
-            P.ldc = I.ldc;
             P.alpha = 1.0;
-            P.beta = 1.0;
             P.cscale = I.dinv + off;
             P.eB = I.eZ + st.eoff;
-            panel_tile<ANPL>(P, i0, 0, raw);
-        } else {
-            // update: B(:, c0:c0+w) -= B(:, k0:k0+kw) * L(c0:c0+w, k0:k0+kw)^T
-            const int c0 = st.a, w = st.b, k0 = st.c, kw = st.d;
-            P.n = w;
-            P.k = kw;
-            P.A = I.C + (long long)k0 * I.ldc;
-            P.lda = I.ldc;
-            P.modeA = kOpN;
-            P.B = I.A + c0 + (long long)k0 * I.lda;
-            P.ldb = I.lda;
-            P.modeB = kOpT;
-            P.C = I.C + (long long)c0 * I.ldc;
-            P.ldc = I.ldc;
-            P.alpha = -1.0;
-            P.beta = 1.0;
-            P.eB = I.eL + c0;
-            if (st.type == 1)
-                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<UNPL>(P, i0, j0, raw);
-            else
-                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<ANPL>(P, i0, j0, raw);
+            panel_tile<ANPL, kPP>(P, i0, 0, raw);
+            continue;
+        }
+        // update: B(:, c0:c0+w) -= B(:, k0:k0+kw) * L(c0:c0+w, k0:k0+kw)^T, in the update
+        // tier (type 1) or, inside a wide leaf block, in the leaf tier (type 2)
+        const int c0 = st.a, w = st.b, k0 = st.c, kw = st.d;
+        P.n = w;
+        P.k = kw;
+        P.C = I.C + (long long)c0 * I.ldc;
+        P.alpha = -1.0;
+        P.eB = I.eL + c0;
+        {
+            // L's plane rows give 16-byte aligned 32-k stages from column k0 (in the copy
+            // shifted by kshift1 when k0 is not a multiple of 8); past kw they meet the
+            // zero padding of X1's planes
+            const bool sh = (k0 & 7) != 0;
+            const long long kb = (long long)c0 * I.ldpL + k0 + (sh ? I.kshift1 : 0);
+            P.A = scrA;
+            P.lda = S.ldp;
+            P.spA = S.sp;
+            P.ldb = I.ldpL;
+            P.srcA = P.srcB = kPP;
+            if (st.type == 1) {
+                panel_planes<UNPL>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap, scr,
+                                   S.ldp, S.sp, t);
+                P.B = static_cast<const _Float16*>(sh ? I.LpU1 : I.LpU) + kb;
+                P.spB = I.spLU;
+                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<UNPL, kPP>(P, i0, j0, raw);
+            } else {
+                panel_planes<ANPL>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap, scr,
+                                   S.ldp, S.sp, t);
+                P.B = static_cast<const _Float16*>(sh ? I.LpA1 : I.LpA) + kb;
+                P.spB = I.spLA;
+                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<ANPL, kPP>(P, i0, j0, raw);
+            }
         }
     }
 }
@@ -108,10 +198,11 @@ __device__ void trsm_panel(const BatchInst& I, const BatchStep* __restrict__ ste
 template <int UNPL, int ANPL, int SQ>
 __global__ void __launch_bounds__(256)
 batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* __restrict__ insts,
-             const BatchStep* __restrict__ steps, int* counter)
+             const BatchStep* __restrict__ steps, int* counter, BatchScratch S)
 {
-    constexpr size_t kLds = cmax(cmax(sizeof(LdsPanel<UNPL>), sizeof(LdsPanel<ANPL>)),
-                                 SQ ? sizeof(LdsSq<UNPL>) : 0);
+    constexpr size_t kPanels = cmax(cmax(sizeof(LdsPanel<UNPL, kPP>), sizeof(LdsPanel<ANPL, kPP>)),
+                                    cmax(sizeof(LdsPanel<UNPL, kFF>), sizeof(LdsPanel<ANPL, kFF>)));
+    constexpr size_t kLds = cmax(cmax(kPanels, SQ ? sizeof(LdsSq<UNPL>) : 0), 64 * 65 * sizeof(float));
     __shared__ __attribute__((aligned(1024))) char raw[kLds];
     __shared__ int next_s;
     for (;;) {
@@ -123,16 +214,16 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
         const BatchItem w = items[it];
         const BatchInst& I = insts[w.inst];
         if (I.op == kBatchTrsm) {
-            trsm_panel<UNPL, ANPL>(I, steps, w.i0, raw);
+            trsm_panel<UNPL, ANPL>(I, steps, w.i0, S, raw);
         } else {
             GemmProb P = {};
             P.m = P.n = I.n;
             P.k = I.k;
-            P.A = P.B = I.A;
-            P.lda = P.ldb = I.lda;
-            P.modeA = kOpN;
-            P.modeB = kOpT;
-            P.srcA = P.srcB = kSrcF32;
+            P.A = P.B = I.Ap;
+            P.lda = P.ldb = I.ldpA;
+            P.spA = P.spB = I.spA;
+            P.modeA = P.modeB = kOpN;
+            P.srcA = P.srcB = kPP;
             P.C = I.C;
             P.ldc = I.ldc;
             P.alpha = I.alpha;
@@ -142,10 +233,10 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
             P.batch = 1;
             if constexpr (SQ) {
                 auto* L = reinterpret_cast<LdsSq<UNPL>*>(raw);
-                gemm16_tile<UNPL, kSrcF32, kSrcF32, 128, 128, 2, 2, 2>(P, 0, w.i0, w.j0, L->ring,
-                                                                       L->scA, L->scB, L->emax);
+                gemm16_tile<UNPL, kPP, kPP, 128, 128, 2, 2, sq_stages<UNPL>()>(
+                    P, 0, w.i0, w.j0, L->ring, L->scA, L->scB, L->emax);
             } else {
-                panel_tile<UNPL>(P, w.i0, w.j0, raw);
+                panel_tile<UNPL, kPP>(P, w.i0, w.j0, raw);
             }
         }
     }
@@ -185,16 +276,33 @@ batch_trtri_kernel(const TrtriJob* __restrict__ jobs, int njobs)
                 J.dinv + (long long)q * J.s, cb, y, lrow);
 }
 
+__global__ void __launch_bounds__(256)
+batch_planes_kernel(const PlanesJob* __restrict__ jobs, int njobs)
+{
+    __shared__ float t[64][65];
+    const PlanesJob& J = find_job(jobs, njobs, blockIdx.x);
+    const int local = blockIdx.x - J.blockBegin;
+    const int bx = local % J.gx, by = local / J.gx;
+    _Float16* out = static_cast<_Float16*>(J.out);
+    if (J.npl == 1)
+        split_planes_block<1>(J.X, J.ld, J.mode, J.rows, J.K, J.e, nullptr, out, J.ldp, J.sp, J.kshift, bx, by, t);
+    else if (J.npl == 2)
+        split_planes_block<2>(J.X, J.ld, J.mode, J.rows, J.K, J.e, nullptr, out, J.ldp, J.sp, J.kshift, bx, by, t);
+    else
+        split_planes_block<3>(J.X, J.ld, J.mode, J.rows, J.K, J.e, nullptr, out, J.ldp, J.sp, J.kshift, bx, by, t);
+}
+
 template <int UNPL, int ANPL>
 hipError_t run_sq(int sq, const BatchItem* items, int nitems, const BatchInst* insts,
-                  const BatchStep* steps, int* counter, int grid, hipStream_t s)
+                  const BatchStep* steps, int* counter, const BatchScratch& S, int grid,
+                  hipStream_t s)
 {
     if (sq)
         hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 1>), dim3(grid), dim3(256), 0, s, items, nitems,
-                           insts, steps, counter);
+                           insts, steps, counter, S);
     else
         hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 0>), dim3(grid), dim3(256), 0, s, items, nitems,
-                           insts, steps, counter);
+                           insts, steps, counter, S);
     return hipGetLastError();
 }
 
@@ -224,16 +332,23 @@ hipError_t batch_trtri(const TrtriJob* jobs, int njobs, int nblocks, hipStream_t
     return hipGetLastError();
 }
 
+hipError_t batch_planes(const PlanesJob* jobs, int njobs, int nblocks, hipStream_t s)
+{
+    if (njobs <= 0 || nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_planes_kernel, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+
 hipError_t batch_run(int unpl, int anpl, int sq, const BatchItem* items, int nitems,
-                     const BatchInst* insts, const BatchStep* steps, int* counter, int grid,
-                     hipStream_t s)
+                     const BatchInst* insts, const BatchStep* steps, int* counter,
+                     const BatchScratch& S, int grid, hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
     if (grid <= 0) return hipErrorInvalidValue;
-    if (unpl == 1 && anpl == 2) return run_sq<1, 2>(sq, items, nitems, insts, steps, counter, grid, s);
-    if (unpl == 1 && anpl == 3) return run_sq<1, 3>(sq, items, nitems, insts, steps, counter, grid, s);
-    if (unpl == 2 && anpl == 2) return run_sq<2, 2>(sq, items, nitems, insts, steps, counter, grid, s);
-    if (unpl == 3 && anpl == 3) return run_sq<3, 3>(sq, items, nitems, insts, steps, counter, grid, s);
+    if (unpl == 1 && anpl == 2) return run_sq<1, 2>(sq, items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 1 && anpl == 3) return run_sq<1, 3>(sq, items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 2 && anpl == 2) return run_sq<2, 2>(sq, items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 3 && anpl == 3) return run_sq<3, 3>(sq, items, nitems, insts, steps, counter, S, grid, s);
     return hipErrorInvalidValue;
 }
 

@@ -30,6 +30,9 @@ struct tb_batch_s {
     int nrj[2] = {0, 0}, rjblocks[2] = {0, 0};
     TrtriJob* tj = nullptr;
     int ntj = 0, tjblocks = 0;
+    PlanesJob* pj = nullptr;
+    int npj = 0, pjblocks = 0;
+    BatchScratch scratch = {nullptr, 0, 0, 0};
     int* exps = nullptr;
     long long nexps = 0;
     int* counter = nullptr;
@@ -81,8 +84,9 @@ hipError_t enqueue(const tb_batch_s* b, hipStream_t s)
     if ((e = batch_rowexp(b->rj[0], b->nrj[0], b->rjblocks[0], s)) != hipSuccess) return e;
     if ((e = batch_trtri(b->tj, b->ntj, b->tjblocks, s)) != hipSuccess) return e;
     if ((e = batch_rowexp(b->rj[1], b->nrj[1], b->rjblocks[1], s)) != hipSuccess) return e;
+    if ((e = batch_planes(b->pj, b->npj, b->pjblocks, s)) != hipSuccess) return e;
     return batch_run(b->unpl, b->anpl, b->sq, b->items[b->sq], b->nitems[b->sq], b->insts, b->steps,
-                     b->counter, b->grid, s);
+                     b->counter, b->scratch, b->grid, s);
 }
 
 void release(tb_batch_s* b)
@@ -115,9 +119,18 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     // workspace offsets, fixed up to pointers after the allocation
     struct Off {
         long long eRow = -1, eSnap = -1, eL = -1, eZ = -1, Z = -1, dinv = -1;
+        long long LpU = -1, LpA = -1, Zp = -1, Ap = -1, ldp = 0, esum = 0;   // planes: halves
+        long long LpU1 = -1, LpA1 = -1;
+        int kshift1 = 0;
     };
     std::vector<Off> off(count);
-    long long nexps = 0, nsnap = 0, nz = 0, ndinv = 0;
+    long long nexps = 0, nsnap = 0, nz = 0, ndinv = 0, nhalves = 0, maxk = 64;
+    auto rup = [](long long v, long long q) { return (v + q - 1) / q * q; };
+    auto reserve_halves = [&](long long h) {
+        const long long o = nhalves;
+        nhalves += rup(h, 128);        // 256-byte aligned
+        return o;
+    };
 
     std::vector<TrsmStep> sched;
     std::vector<TrsmLeaf> leaves;
@@ -143,6 +156,8 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
                 off[i].eRow = nexps;
                 nexps += e.n;
             }
+            off[i].ldp = rup(std::max(1l, e.k), 64);
+            off[i].Ap = reserve_halves((long long)b->unpl * e.n * off[i].ldp);
             // 128 x 128 lower tiles (variant 1) and 64 x 256 tiles meeting the lower part (0)
             const long t1 = (e.n + 127) / 128;
             for (long ti = 0; ti < t1; ++ti)
@@ -171,7 +186,13 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
             ez += leaves[q].s;
         }
         size_t lq = 0;
+        int resid = -1;        // the one residue mod 8 of the unaligned update offsets
         for (const TrsmStep& s : sched) {
+            maxk = std::max<long long>(maxk, s.type == 0 ? s.b : s.d);
+            if (s.type != 0 && s.c % 8 != 0) {
+                if (resid >= 0 && resid != (int)(s.c % 8)) return TB_ERR_INVALID;   // not from this planner
+                resid = (int)(s.c % 8);
+            }
             BatchStep st;
             std::memset(&st, 0, sizeof(st));
             st.type = s.type == 0 ? 0 : (s.acc ? 2 : 1);
@@ -198,6 +219,18 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         nz += z;
         off[i].dinv = ndinv;
         ndinv += e.n;
+        // L's rows in planes: 32 columns of slack past n (and past the shift), so that a
+        // 32-k stage starting inside the row never leaves it
+        off[i].kshift1 = resid > 0 ? 8 - resid : 0;
+        off[i].ldp = rup(e.n + off[i].kshift1 + 32, 64);
+        off[i].esum = ez;
+        off[i].LpU = reserve_halves((long long)b->unpl * e.n * off[i].ldp);
+        off[i].LpA = b->unpl == 1 ? reserve_halves((long long)b->anpl * e.n * off[i].ldp) : off[i].LpU;
+        if (resid > 0) {
+            off[i].LpU1 = reserve_halves((long long)b->unpl * e.n * off[i].ldp);
+            off[i].LpA1 = b->unpl == 1 ? reserve_halves((long long)b->anpl * e.n * off[i].ldp) : off[i].LpU1;
+        }
+        off[i].Zp = reserve_halves((long long)b->anpl * ez * kZpLd);
         for (long r0 = 0; r0 < e.m; r0 += kPanelRows) {
             const double rows = (double)std::min<long>(kPanelRows, e.m - r0);
             const Costed c = {{i, (int)r0, 0, 0}, rows * (double)e.n * (double)e.n * 0.5};
@@ -221,6 +254,12 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     const size_t oRj0 = ar.reserve(sizeof(RowexpJob) * (2 * (size_t)count + 1));
     const size_t oRj1 = ar.reserve(sizeof(RowexpJob) * (size_t)std::max<size_t>(1, steps.size() + 1));
     const size_t oTj = ar.reserve(sizeof(TrtriJob) * (size_t)std::max<size_t>(1, steps.size() + 1));
+    const size_t oPj = ar.reserve(sizeof(PlanesJob) * (4 * (size_t)count + steps.size() + 1));
+    const size_t oPl = ar.reserve(sizeof(_Float16) * (size_t)std::max<long long>(1, nhalves));
+    // per-workgroup scratch: a panel's solved block in the leaf tier's planes
+    const long long sldp = rup(maxk, 64);
+    const long long sper = rup((long long)std::max(b->unpl, b->anpl) * kPanelRows * sldp * 2, 256);
+    const size_t oScr = ar.reserve((size_t)b->grid * sper);
     if (hipMalloc(&b->dev, ar.total()) != hipSuccess) {
         (void)hipGetLastError();
         b->dev = nullptr;
@@ -241,6 +280,32 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     b->rj[0] = reinterpret_cast<RowexpJob*>(d + oRj0);
     b->rj[1] = reinterpret_cast<RowexpJob*>(d + oRj1);
     b->tj = reinterpret_cast<TrtriJob*>(d + oTj);
+    b->pj = reinterpret_cast<PlanesJob*>(d + oPj);
+    _Float16* planes = reinterpret_cast<_Float16*>(d + oPl);
+    b->scratch = {d + oScr, sper, sldp, kPanelRows * sldp};
+    std::vector<PlanesJob> pj;
+    auto add_planes = [&](const float* X, long long ld, int mode, long rows, long K, const int* ex,
+                          int npl, _Float16* out, long long ldp, long long sp, int kshift = 0) {
+        if (rows <= 0) return;
+        PlanesJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.X = X;
+        j.ld = ld;
+        j.mode = mode;
+        j.rows = (int)rows;
+        j.K = (int)K;
+        j.npl = npl;
+        j.kshift = kshift;
+        j.e = ex;
+        j.out = out;
+        j.ldp = ldp;
+        j.sp = sp;
+        j.gx = (int)(ldp / 64);
+        j.gy = (int)((rows + 63) / 64);
+        j.blockBegin = b->pjblocks;
+        b->pjblocks += j.gx * j.gy;
+        pj.push_back(j);
+    };
 
     auto add_rowexp = [](std::vector<RowexpJob>& v, int& blocks, const float* X, long long ld,
                          int mode, long rows, long K, int* out) {
@@ -267,6 +332,13 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         if (o.eRow >= 0) I.eRow = b->exps + o.eRow;
         if (e.op == TB_OP_SYRK) {
             if (o.eRow >= 0) add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpN, e.n, e.k, I.eRow);
+            if (e.n == 0) continue;
+            // the rows of A in planes, as the single call converts them
+            I.Ap = planes + o.Ap;
+            I.ldpA = o.ldp;
+            I.spA = (long long)e.n * o.ldp;
+            add_planes(e.A, e.lda, kOpN, e.n, e.k, split ? I.eRow : nullptr, b->unpl, planes + o.Ap,
+                       o.ldp, I.spA);
             continue;
         }
         if (e.m == 0 || e.n == 0) continue;
@@ -278,6 +350,28 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         // exponents of B's rows and L's rows (split updates), as trsm_run scans them
         add_rowexp(rj[0], b->rjblocks[0], e.C, e.ldc, kOpN, e.m, e.n, I.eRow);
         add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpTriL, e.n, e.n, (int*)I.eL);
+        // L's rows in the update tier's planes and, if it differs, the leaf tier's
+        I.ldpL = o.ldp;
+        I.spLU = (long long)e.n * o.ldp;
+        I.spLA = I.spLU;
+        I.LpU = planes + o.LpU;
+        I.LpA = planes + o.LpA;
+        add_planes(e.A, e.lda, kOpTriL, e.n, e.n, split ? I.eL : nullptr, b->unpl, planes + o.LpU,
+                   o.ldp, I.spLU);
+        if (o.LpA != o.LpU)
+            add_planes(e.A, e.lda, kOpTriL, e.n, e.n, I.eL, b->anpl, planes + o.LpA, o.ldp, I.spLA);
+        I.kshift1 = o.kshift1;
+        I.LpU1 = o.LpU1 >= 0 ? planes + o.LpU1 : I.LpU;
+        I.LpA1 = o.LpA1 >= 0 ? planes + o.LpA1 : I.LpA;
+        if (o.LpU1 >= 0) {
+            add_planes(e.A, e.lda, kOpTriL, e.n, e.n, split ? I.eL : nullptr, b->unpl,
+                       planes + o.LpU1, o.ldp, I.spLU, o.kshift1);
+            if (o.LpA1 != o.LpU1)
+                add_planes(e.A, e.lda, kOpTriL, e.n, e.n, I.eL, b->anpl, planes + o.LpA1, o.ldp,
+                           I.spLA, o.kshift1);
+        }
+        I.Zp = planes + o.Zp;
+        I.spZ = o.esum * kZpLd;
         // inverses: one job per run of equal adjacent leaves, as trsm_run launches them
         trsm_schedule(e.n, nb, &sched, &leaves);
         long long z = 0, ez = 0;
@@ -300,9 +394,12 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
             t.blockBegin = b->tjblocks;
             b->tjblocks += t.colBlocks * t.count;
             tj.push_back(t);
-            // exponents of the columns of the run's W^T blocks (stacked rows of Z^T)
+            // exponents of the columns of the run's W^T blocks (stacked rows of Z^T), and
+            // those rows in the leaf tier's planes
             add_rowexp(rj[1], b->rjblocks[1], Z + o.Z + z, s, kOpT, s * (long)(r - q), s,
                        (int*)I.eZ + ez);
+            add_planes(Z + o.Z + z, s, kOpT, s * (long)(r - q), s, I.eZ + ez, b->anpl,
+                       planes + o.Zp + ez * kZpLd, kZpLd, I.spZ);
             z += s * s * (long long)(r - q);
             ez += s * (long long)(r - q);
             q = r;
@@ -311,9 +408,12 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     b->nrj[0] = (int)rj[0].size();
     b->nrj[1] = (int)rj[1].size();
     b->ntj = (int)tj.size();
+    b->npj = (int)pj.size();
     if (rj[0].size() > 2 * (size_t)count + 1 || rj[1].size() > steps.size() + 1 ||
-        tj.size() > steps.size() + 1)
+        tj.size() > steps.size() + 1 || pj.size() > 4 * (size_t)count + steps.size() + 1)
         return TB_ERR_INVALID;   // cannot happen: each leaf run is one step at least
+    if (!pj.empty())
+        TB_HIP(hipMemcpy(b->pj, pj.data(), sizeof(PlanesJob) * pj.size(), hipMemcpyHostToDevice));
 
     for (int v = 0; v < 2; ++v) {
         sort_items(items[v]);

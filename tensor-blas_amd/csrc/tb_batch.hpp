@@ -35,6 +35,26 @@ struct BatchInst {
     const float* Z;              // TRSM: W^T = (D inv(L_jj) - I)^T of each leaf, s x s, stacked
     const double* dinv;          // TRSM: 1 / L(j, j)
     int step0, nsteps;           // TRSM: its steps in the step table
+    // FP16 planes (K-contiguous rows, plane stride sp) made by the prologue:
+    const void* LpU;             // TRSM: rows of L in the update tier's planes (ldpL)
+    const void* LpA;             // TRSM: rows of L in the leaf tier's planes
+    const void* LpU1;            // TRSM: the same shifted right by kshift1 columns, for
+    const void* LpA1;            //   updates whose column offset is not a multiple of 8
+    long long ldpL, spLU, spLA;
+    int kshift1;
+    const void* Zp;              // TRSM: rows of every leaf's W^T (leaf tier), ld kZpLd
+    long long spZ;
+    const void* Ap;              // SYRK: rows of A
+    long long ldpA, spA;
+};
+
+constexpr int kZpLd = 256;       // plane row length of the leaves' W^T (>= every leaf width)
+
+// per-workgroup scratch of the persistent kernel: a panel's solved block in planes
+struct BatchScratch {
+    void* base;                  // workgroup b uses base + b * bytes_per_wg
+    long long bytes_per_wg;
+    long long ldp, sp;           // plane row length and plane stride (64 rows)
 };
 
 struct BatchStep {
@@ -61,6 +81,16 @@ struct RowexpJob {
     int lower;                   // kOpTriL: only k <= r
     int* out;
     int blockBegin, rowBlocks, kBlocks;
+};
+
+struct PlanesJob {               // one FP32 -> FP16 plane conversion (split_planes)
+    const float* X;
+    long long ld;
+    int mode, rows, K, npl, kshift;
+    const int* e;                // row exponents (split planes), else null
+    void* out;                   // plane 0; planes at out + p * sp (halves)
+    long long ldp, sp;
+    int blockBegin, gx, gy;      // gx = ldp / 64 plane-column blocks, gy = ceil(rows / 64)
 };
 
 struct TrtriJob {

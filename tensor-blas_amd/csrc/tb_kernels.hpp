@@ -112,19 +112,22 @@ hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s);
 // ---------------------------------------------------------------------- batches
 struct RowexpJob;
 struct TrtriJob;
+struct PlanesJob;
 struct BatchItem;
 struct BatchInst;
 struct BatchStep;
+struct BatchScratch;
 // row exponents / diagonal-block inverses of many operands in one launch each
 // (nblocks = the jobs' summed block counts)
 hipError_t batch_rowexp(const RowexpJob* jobs, int njobs, int nblocks, hipStream_t s);
 hipError_t batch_trtri(const TrtriJob* jobs, int njobs, int nblocks, hipStream_t s);
+hipError_t batch_planes(const PlanesJob* jobs, int njobs, int nblocks, hipStream_t s);
 // the persistent batch kernel: unpl / anpl = FP16 planes of update (and SYRK) tiles /
-// leaf tiles, sq = SYRK tiles of 128 x 128 (1) or 64 x 256 (0); grid = workgroups;
-// counter must hold 0
+// leaf tiles, sq = SYRK tiles of 128 x 128 (1) or 64 x 256 (0); grid = workgroups
+// (each with its scratch in S); counter must hold 0
 hipError_t batch_run(int unpl, int anpl, int sq, const BatchItem* items, int nitems,
-                     const BatchInst* insts, const BatchStep* steps, int* counter, int grid,
-                     hipStream_t s);
+                     const BatchInst* insts, const BatchStep* steps, int* counter,
+                     const BatchScratch& S, int grid, hipStream_t s);
 // workgroups of the persistent batch kernel that one device runs at once
 int batch_grid(int device);
 

@@ -28,7 +28,7 @@ def h(tbm):
 
 
 SYRKS = [(1, 1), (7, 33), (64, 64), (130, 70), (300, 300), (513, 200)]      # (n, k)
-TRSMS = [(1, 1), (64, 64), (100, 300), (257, 1000), (65, 2048), (300, 513)]  # (m, n)
+TRSMS = [(1, 1), (64, 64), (100, 300), (257, 1000), (65, 2048), (300, 513), (70, 3001)]  # (m, n)
 
 
 def make_problems(seed):
