@@ -71,33 +71,79 @@ __device__ __forceinline__ void rowexp_k_block(const T* __restrict__ X, long lon
 
 // Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.
 // Writes W = D inv(L) - I = (I+N)^-1 - I (strictly lower) as Z = W^T (Z[c + r*s] =
-// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One wave (blockDim 64) owns the
-// 64 columns cb.. of inv(L): lane c computes column cb+c by forward substitution,
-// keeping the column in LDS.  Row r of L is staged in LDS by the whole wave first,
-// so the inner product reads it as a broadcast.  y: 256*64 doubles, lrow: 256.
+// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One 64-lane block owns the
+// kTrtriCols columns cb.. of inv(L): four lanes per column compute it by forward
+// substitution, each summing every fourth term of a row's dot product (combined by
+// shuffles); the columns live in LDS (y: 256 x kTrtriCols doubles) and row r of L is
+// staged in LDS (lrow: 256) so the inner products read it as a broadcast.  The rows of
+// L do not depend on the results, so each lane loads them kTrtriAhead rows early into
+// a register ring: the row loads' latency overlaps the substitution (s <= 256).  The
+// block is one wave, so LDS results are shared by waiting for the wave's own LDS
+// operations (no barrier, whose fence would also wait for the prefetched rows).  A
+// staged-chunk variant (32 rows of L per column, line-contiguous) measured slower: its
+// larger LDS footprint halves the blocks resident per CU.
+constexpr int kTrtriAhead = 8;
+#define TB_WAVE_LDS_SYNC() asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
+
 template <typename Tin, typename Tout>
 __device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long long lda, int s,
                                             Tout* __restrict__ Zq, double* __restrict__ dinvq,
                                             int cb, double* y, double* lrow)
 {
     const int lane = threadIdx.x;
-    const int col = cb + lane;
+    const int cl = lane >> 2, part = lane & 3;
+    const int col = cb + cl;
     const bool live = col < s;
     if (cb == 0)
         for (int r = lane; r < s; r += 64) dinvq[r] = 1.0 / (double)Lq[r + (long long)r * lda];
-    for (int r = 0; r < cb; ++r)
+    for (int r = part; r < cb; r += 4)
         if (live) Zq[col + (long long)r * s] = Tout(0);
-    for (int r = cb; r < s; ++r) {
-        for (int p = cb + lane; p <= r; p += 64) lrow[p - cb] = (double)Lq[r + (long long)p * lda];
-        __syncthreads();
-        double sum = (r == col) ? 1.0 : 0.0;
-        for (int p = cb; p < r; ++p) sum -= lrow[p - cb] * y[(p - cb) * 64 + lane];
-        const double d = lrow[r - cb];
-        const double v = (r < col || !live) ? 0.0 : sum / d;
-        y[(r - cb) * 64 + lane] = v;
-        // W(r, col) = L(r,r) * inv(L)(r, col) for r > col
-        if (live) Zq[col + (long long)r * s] = (r > col) ? (Tout)(d * v) : Tout(0);
-        __syncthreads();
+    // row r of L, columns cb + lane + 64 j (j < 4) with column <= r
+    Tin pre[kTrtriAhead][4];
+    auto fetch = [&](int r, Tin* v) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = cb + lane + 64 * j;
+            v[j] = (r < s && c <= r) ? Lq[r + (long long)c * lda] : Tin(0);
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < kTrtriAhead; ++d) fetch(cb + d, pre[d]);
+    for (int r0 = cb; r0 < s; r0 += kTrtriAhead) {
+#pragma unroll
+        for (int d = 0; d < kTrtriAhead; ++d) {
+            const int r = r0 + d;
+            if (r >= s) break;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = cb + lane + 64 * j;
+                if (c <= r) lrow[c - cb] = (double)pre[d][j];
+            }
+            fetch(r + kTrtriAhead, pre[d]);
+            TB_WAVE_LDS_SYNC();
+            // every fourth term of the row's dot product, in four independent partial
+            // sums so that several LDS reads are in flight at once
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int p = cb + part;
+            for (; p + 12 < r; p += 16) {
+                s0 += lrow[p - cb] * y[(p - cb) * kTrtriCols + cl];
+                s1 += lrow[p + 4 - cb] * y[(p + 4 - cb) * kTrtriCols + cl];
+                s2 += lrow[p + 8 - cb] * y[(p + 8 - cb) * kTrtriCols + cl];
+                s3 += lrow[p + 12 - cb] * y[(p + 12 - cb) * kTrtriCols + cl];
+            }
+            for (; p < r; p += 4) s0 += lrow[p - cb] * y[(p - cb) * kTrtriCols + cl];
+            double sum = (s0 + s1) + (s2 + s3);
+            sum += __shfl_xor(sum, 1);
+            sum += __shfl_xor(sum, 2);
+            const double dg = lrow[r - cb];
+            const double v = (r < col || !live) ? 0.0 : (((r == col) ? 1.0 : 0.0) - sum) / dg;
+            if (part == 0) {
+                y[(r - cb) * kTrtriCols + cl] = v;
+                // W(r, col) = L(r,r) * inv(L)(r, col) for r > col
+                if (live) Zq[col + (long long)r * s] = (r > col) ? (Tout)(dg * v) : Tout(0);
+            }
+            TB_WAVE_LDS_SYNC();
+        }
     }
 }
 

@@ -40,10 +40,10 @@ __global__ void __launch_bounds__(64)
 trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride, int s,
                      Tout* __restrict__ Z, double* __restrict__ dinv)
 {
-    __shared__ double y[256 * 64];
+    __shared__ double y[256 * kTrtriCols];
     __shared__ double lrow[256];
     trtri_block(L + (long long)blockIdx.y * bstride, lda, s, Z + (long long)blockIdx.y * s * s,
-                dinv + (long long)blockIdx.y * s, blockIdx.x * 64, y, lrow);
+                dinv + (long long)blockIdx.y * s, blockIdx.x * kTrtriCols, y, lrow);
 }
 
 // FP32 -> FP16, round to nearest even
@@ -218,7 +218,7 @@ hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda
 {
     if (s <= 0 || nblocks <= 0) return hipSuccess;
     if (s > 256) return hipErrorInvalidValue;
-    dim3 g((s + 63) / 64, nblocks);
+    dim3 g((s + kTrtriCols - 1) / kTrtriCols, nblocks);
     if (in_f64 && out_f64)
         hipLaunchKernelGGL((trtri_lower_t_kernel<double, double>), g, dim3(64), 0, st,
                            (const double*)L, lda, bstride, s, (double*)Z, dinv);

@@ -390,7 +390,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
             t.count = (int)(r - q);
             t.Z = Z + o.Z + z;
             t.dinv = dinv + o.dinv + lo;
-            t.colBlocks = (int)((s + 63) / 64);
+            t.colBlocks = (int)((s + kTrtriCols - 1) / kTrtriCols);
             t.blockBegin = b->tjblocks;
             b->tjblocks += t.colBlocks * t.count;
             tj.push_back(t);
