@@ -125,14 +125,44 @@ int GemmQueue::add(const GemmProb& p0)
     return TB_SUCCESS;
 }
 
+// Square launches of full-C plane products with at least one wide tile per CU run on
+// 256 x 128 tiles instead (fewer LDS bytes per MFMA); smaller launches keep 128 x 128,
+// whose twice as many tiles fill more of the device.
+bool GemmQueue::retile_wide()
+{
+    const int npl = planes_of(h_, tier_);
+    if (shape_ != kShapeSquare || srcA_ != kSrcPlanes || srcB_ != kSrcPlanes || npl < 1 || npl > 2)
+        return false;
+    const TileShape w = gemm_tile_shape(tier_, kShapeWide);
+    long long tiles = 0;
+    for (int q = 0; q < L_.np; ++q) {
+        const GemmProb& p = L_.p[q];
+        if (p.modeC == kCLower) return false;
+        tiles += (long long)((p.m + w.bm - 1) / w.bm) * ((p.n + w.bn - 1) / w.bn) * p.batch;
+    }
+    if (tiles < kWideMinTiles) return false;
+    int begin = 0;
+    for (int q = 0; q < L_.np; ++q) {
+        GemmProb& p = L_.p[q];
+        p.tilesM = (p.m + w.bm - 1) / w.bm;
+        p.tilesN = (p.n + w.bn - 1) / w.bn;
+        p.tilesPer = p.tilesM * p.tilesN;
+        p.tileBegin = begin;
+        begin += p.tilesPer * p.batch;
+    }
+    L_.totalTiles = begin;
+    return true;
+}
+
 int GemmQueue::flush()
 {
     if (L_.np == 0) return TB_SUCCESS;
     double flops = 0.0;
     for (int q = 0; q < L_.np; ++q) flops += prob_flops(L_.p[q]);
+    const int shape = retile_wide() ? kShapeWide : shape_;
     hipEvent_t e0;
     TB_TRY(prof_begin(h_, &e0));
-    hipError_t e = gemm_launch(tier_, planes_of(h_, tier_), shape_, srcA_, srcB_, L_, h_->stream);
+    hipError_t e = gemm_launch(tier_, planes_of(h_, tier_), shape, srcA_, srcB_, L_, h_->stream);
     std::memset(&L_, 0, sizeof(L_));
     TB_HIP(e);
     return prof_end(h_, kProfGemm0 + tier_ * 2 + shape_, e0, flops);

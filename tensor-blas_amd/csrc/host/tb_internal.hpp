@@ -86,8 +86,10 @@ public:
     GemmQueue(tb_handle_s* h, int tier, int shape, int srcA = 0, int srcB = 0);
     int add(const GemmProb& p);
     int flush();
+    static constexpr long long kWideMinTiles = 256;   // one wide tile per CU
 
 private:
+    bool retile_wide();
     tb_handle_s* h_;
     int tier_, shape_, srcA_, srcB_;
     TileShape ts_;
