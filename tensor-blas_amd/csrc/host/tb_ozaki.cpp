@@ -30,10 +30,12 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     Arena a;
     const size_t oEA = a.reserve(sizeof(int) * m);
     const size_t oEB = a.reserve(sizeof(int) * n);
-    const size_t oSA = a.reserve((size_t)S * m * kpad);
-    const size_t oSB = a.reserve((size_t)S * n * kpad);
-    const size_t oSA2 = pairs ? a.reserve((size_t)S * m * kpad) : 0;
-    const size_t oSB2 = pairs ? a.reserve((size_t)S * n * kpad) : 0;
+    // slices hold rows padded to a multiple of 64
+    const size_t mp = (size_t)(m + 63) / 64 * 64, np = (size_t)(n + 63) / 64 * 64;
+    const size_t oSA = a.reserve((size_t)S * mp * kpad);
+    const size_t oSB = a.reserve((size_t)S * np * kpad);
+    const size_t oSA2 = pairs ? a.reserve((size_t)S * mp * kpad) : 0;
+    const size_t oSB2 = pairs ? a.reserve((size_t)S * np * kpad) : 0;
     char* ws;
     TB_TRY(workspace_get(h, a.total(), &ws, 1));
     hipStream_t st = h->stream;

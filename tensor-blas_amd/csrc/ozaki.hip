@@ -29,6 +29,7 @@ typedef int intx4_t __attribute__((ext_vector_type(4)));
 typedef int intx16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kSliceTile = 64;
+constexpr int kOzBlockK = 32;        // k per block of the slice layout (= the GEMM's stage)
 
 // Slices of a 64 x 64 block of op(X) rows: loaded through LDS so that both the
 // source read and the int8 write are coalesced.
@@ -57,11 +58,12 @@ ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int row
         t[rl][kl] = v;
     }
     __syncthreads();
-    // each thread: one row, 16 consecutive k
+    // each thread: one row, 16 consecutive k; rows past `rows` (up to the next multiple
+    // of 64) get zero digits
     const int rl = tid >> 2, kl0 = (tid & 3) * 16;
     const int r = r0 + rl;
-    if (r >= rows || k0 + kl0 >= kpad) return;
-    const int e = exps[(long long)q * rows + r];
+    if (k0 + kl0 >= kpad) return;
+    const int e = r < rows ? exps[(long long)q * rows + r] : 0;
     const int sh = (e < -5000) ? 0 : e;   // all-zero row: digits are zero anyway
     double rem[16];
 #pragma unroll
@@ -81,24 +83,40 @@ ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int row
             }
             pk[w] = (int)word;
         }
-        int8_t* dst = out + (((long long)q * S + s) * rows + r) * (long long)kpad + k0 + kl0;
+        // panel-blocked layout: 32 k of all S slices of one 64-row block are contiguous
+        const int k = k0 + kl0;
+        const long long rblocks = (rows + kSliceTile - 1) / kSliceTile;
+        int8_t* dst = out + (long long)q * S * rblocks * kSliceTile * kpad +
+                      (((long long)(k / kOzBlockK) * rblocks + blockIdx.y) * S + s) * (kSliceTile * kOzBlockK) +
+                      rl * kOzBlockK + k % kOzBlockK;
         *reinterpret_cast<intx4_t*>(dst) = pk;
     }
 }
 
-constexpr int OBM = 64, OBN = 64, OBK = 64;   // tile and k block (64 int8 = one LDS row)
+constexpr int OBM = 64, OBN = 64;   // C tile
+constexpr int OBK = kOzBlockK;       // k per stage: one 32-byte LDS row of int8 per slice row
 constexpr int kMinSlices = 3, kMaxSlices = 10;
-constexpr int kOzStages = 2;
+
+// LDS ring depth for S slices: as many stages (<= 4) of 2*S*64 rows x 32 B as fit
+// below 160 KB
+template <int S>
+constexpr int oz_stages()
+{
+    return 4 * (2 * S * OBM * OBK) < 160 * 1024 ? 4 : 3;
+}
 
 // direct-to-LDS load of 16 bytes per lane (see gemm16_kernel.hpp lds_dma16)
 __device__ __forceinline__ void oz_dma16(const void* g, unsigned lds_base)
 {
     asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
                  :
-                 : "s"(lds_base), "v"(g)
+                 : "s"(__builtin_amdgcn_readfirstlane(lds_base)), "v"(g)
                  : "memory", "m0");
 }
-#define TB_OZ_WAIT_BARRIER() asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory")
+// "s_waitcnt vmcnt(n)" (this wave's direct-to-LDS loads but the newest n are done),
+// lgkmcnt(0), then a workgroup barrier
+#define TB_OZ_WAIT_BARRIER(n) \
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"i"(n) : "memory")
 
 // Lower-triangular tile g of a T x T tile grid, walked in bands of kBand tile rows:
 // inside a band column by column (rows fastest), first the columns left of the band,
@@ -136,18 +154,25 @@ __device__ __forceinline__ void lower_tile_banded(int g, int T, int& ti, int& tj
     ti = tj = T - 1;                                              // unreachable for g < total
 }
 
-// One 64 x 64 tile of C per workgroup; each wave owns a 32 x 32 block.  A k block
-// holds the 64-byte rows of all S slices of both panels (S * 8 KB) in LDS, the four
-// 16-byte units of row r at unit ^ ((r >> 2) & 3); kOzStages blocks are in flight,
-// block kb+1's loads overlapping block kb's MFMAs.
+// One 64 x 64 tile of C per workgroup; each wave owns a 32 x 32 block.  A stage holds
+// 32 k of all S slices of both 64-row panels (S * 4 KB) in LDS as 32-byte rows whose
+// two 16-byte units swap places in every other group of 8 rows (unit ^ ((r >> 3) & 1)),
+// so both the fragment reads (16 lanes: 16 rows of one unit) and the 1 KB direct-to-LDS
+// writes cover the 64 banks once.  The ring holds NST stages; each stage's loads are
+// issued NST-1 stages ahead of its MFMAs.
 template <int S>
 __global__ void __launch_bounds__(256)
 ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
 {
+    constexpr int NST = oz_stages<S>();
     constexpr int PLANE = OBM * OBK;                 // bytes of one slice of one panel
     constexpr int STAGE = 2 * S * PLANE;             // A slices, then B slices
-    static_assert(kOzStages * STAGE <= 160 * 1024, "LDS");
-    __shared__ __attribute__((aligned(1024))) int8_t smem[kOzStages * STAGE];
+    static_assert(NST * STAGE <= 160 * 1024, "LDS");
+    // direct-to-LDS instructions per wave per stage (1 KB each): the vmcnt of a stage
+    constexpr int DMA_W = STAGE / 1024 / 4;
+    constexpr int KEEP = (NST - 2) * DMA_W;
+    static_assert(DMA_W == S && KEEP <= 60, "vmcnt");
+    __shared__ __attribute__((aligned(1024))) int8_t smem[NST * STAGE];
 
     const int g = xcd_remap(blockIdx.x, totalTiles);
     int ti, tj;
@@ -168,29 +193,31 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
 
-    // this wave's direct-to-LDS loads: instruction q moves 16 rows (1 KB) of one slice
-    // of one panel; 8*S instructions per block, 2*S per wave
-    constexpr int NQ = 2 * S;
-    const int8_t* gsrc[NQ];
-    unsigned ldo[NQ];
+    // this wave's direct-to-LDS loads: instruction q moves 32 rows (1 KB) of one slice of
+    // one panel; 4*S instructions per stage, S per wave.  In the panel-blocked slice
+    // layout a panel's stage (S slices x 64 rows x 32 k) is one contiguous block laid
+    // out as in LDS, and stage kb of a panel is kb * (row blocks) * S * PLANE further.
+    const int8_t* gsrc[DMA_W];
+    bool isB[DMA_W];
+    unsigned ldo[DMA_W];
+    const long long kstepA = (long long)((P.m + OBM - 1) / OBM) * S * PLANE;
+    const long long kstepB = (long long)((P.n + OBN - 1) / OBN) * S * PLANE;
     int pair_src = -1;
     const unsigned smem_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)smem;
     auto dma_setup = [&](int pair) {
         const int8_t* GA = pair ? P.sa2 : P.sa;
         const int8_t* GB = pair ? P.sb2 : P.sb;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int inst = wave + q * 4;                  // 0 .. 8S-1
-            const int panel = inst / (4 * S);               // 0: A, 1: B
-            const int rest = inst % (4 * S);
-            const int sl = rest / 4, blk = rest % 4;
-            const int pos = blk * 64 + lane;                // physical unit in the slice
-            const int rl = pos >> 2;
-            const int u = (pos & 3) ^ ((rl >> 2) & 3);      // logical unit stored there
-            const int rows = panel ? P.n : P.m;
-            int r = (panel ? j0 : i0) + rl;
-            r = r < rows ? r : rows - 1;
-            gsrc[q] = (panel ? GB : GA) + ((long long)sl * rows + r) * P.kpad + u * 16;
+        for (int q = 0; q < DMA_W; ++q) {
+            const int inst = wave + q * 4;                  // 0 .. 4S-1
+            const int panel = inst / (2 * S);               // 0: A, 1: B
+            const int rest = inst % (2 * S);
+            const int sl = rest / 2, blk = rest % 2;
+            const int rl = blk * 32 + (lane >> 1);          // LDS row written by this lane
+            const int u = (lane & 1) ^ ((rl >> 3) & 1);     // logical unit stored there
+            const int tile = panel ? tj : ti;               // 64-row block of the operand
+            gsrc[q] = (panel ? GB : GA) + ((long long)tile * S + sl) * PLANE + rl * OBK + u * 16;
+            isB[q] = panel != 0;
             ldo[q] = (unsigned)((panel * S + sl) * PLANE + blk * 1024);
         }
     };
@@ -204,9 +231,9 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
             pair_src = pair;
         }
         const int kb = b - pair * kblocks;
-        const unsigned base = smem_lds + (unsigned)((b % kOzStages) * STAGE);
+        const unsigned base = smem_lds + (unsigned)((b % NST) * STAGE);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) oz_dma16(gsrc[q] + kb * OBK, base + ldo[q]);
+        for (int q = 0; q < DMA_W; ++q) oz_dma16(gsrc[q] + kb * (isB[q] ? kstepB : kstepA), base + ldo[q]);
     };
 
     intx16_t acc[S];
@@ -217,29 +244,38 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
     for (int q = 0; q < S; ++q) acc[q] = intx16_t{};
 
     const int cblocks = P.kchunk / OBK;          // k blocks per exactness chunk
-    const int fr = lane & 31, fsw = (fr >> 2) & 3, fu = lane >> 5;
-    issue(0);
+    // fragment of lane: row (panel base + lane%32), logical unit lane/32
+    const int fr = lane & 31;
+    const int fo = fr * OBK + (((lane >> 5) ^ ((fr >> 3) & 1)) * 16);
+    const int offA = wm * 32 * OBK + fo, offB = S * PLANE + wn * 32 * OBK + fo;
+
+    if (nblk > 0) {
+        for (int b = 0; b < NST - 1; ++b)
+            if (b < nblk) issue(b);
+        // stage 0 complete: only the other issued stages (DMA_W loads each) may be in flight
+        const int issued = nblk < NST - 1 ? nblk : NST - 1;
+        if (issued == 3) TB_OZ_WAIT_BARRIER(2 * DMA_W);
+        else if (issued == 2) TB_OZ_WAIT_BARRIER(DMA_W);
+        else TB_OZ_WAIT_BARRIER(0);
+    }
+#pragma unroll 1
     for (int b = 0; b < nblk; ++b) {
-        TB_OZ_WAIT_BARRIER();                    // block b landed; block b-1's slot is free
-        if (b + 1 < nblk) issue(b + 1);
-        const int8_t* sA = smem + (b % kOzStages) * STAGE;
-        const int8_t* sB = sA + S * PLANE;
+        const int8_t* st = smem + (b % NST) * STAGE;
+        intx4_t af[S], bf[S];
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {         // two 32-k steps = units (2kk, 2kk+1)
-            const int uo = ((2 * kk + fu) ^ fsw) * 16;
-            intx4_t af[S], bf[S];
+        for (int s = 0; s < S; ++s) {
+            af[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offA);
+            bf[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offB);
+        }
+        // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
+        const bool ahead = b + NST - 1 < nblk;
+        if (ahead) issue(b + NST - 1);
+        // D_g += A_s B_{g-s}: consecutive MFMAs update different D_g
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                af[s] = *reinterpret_cast<const intx4_t*>(sA + s * PLANE + (wm * 32 + fr) * OBK + uo);
-                bf[s] = *reinterpret_cast<const intx4_t*>(sB + s * PLANE + (wn * 32 + fr) * OBK + uo);
-            }
-            // D_g += A_s B_{g-s}: consecutive MFMAs update different D_g
+        for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-#pragma unroll
-                for (int gg = s; gg < S; ++gg)
-                    acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
-            }
+            for (int gg = s; gg < S; ++gg)
+                acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
         }
         const int kb = b % kblocks;
         if ((kb + 1) % cblocks == 0 || kb + 1 == kblocks) {
@@ -252,6 +288,9 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
                 acc[gg] = intx16_t{};
             }
         }
+        // block b+1 complete in LDS (later stages may still be in flight)
+        if (ahead) TB_OZ_WAIT_BARRIER(KEEP);
+        else TB_OZ_WAIT_BARRIER(0);
     }
 
     // epilogue: D[jj][ii], ii = lane%32 runs along C's rows

@@ -14,6 +14,9 @@ enum GemmShape : int {
     kShapeSquare = 0,  // 128 x 128 tiles: off-diagonal updates and whole routines
     kShapeLeaf = 1,    // 64 x 256 tiles: a workgroup owns all n <= 256 columns of its rows,
                        // so C may alias op(A) (in-place TRSM leaf X := B * inv(L)^T)
+    kShapeWide = 2,    // 256 x 128 tiles (128 x 64 per wave): full C, both operands in
+                       // FP16 planes, plain or 2-plane split; chosen by GemmQueue::flush
+                       // for launches with enough tiles to fill the device
 };
 struct TileShape {
     int bm, bn;
@@ -90,15 +93,18 @@ hipError_t add_matrix_f32(long long m, long long n, void* x, long long ldx, cons
 
 // ---------------------------------------------------------------------- Ozaki
 // Slices of the rows of op(X) (FP64), 7-bit digits (oracle/oracle.c orc_ozaki_slice):
-// out layout slices[(q*S + s)*rows*kpad + r*kpad + p] int8 (k padded with zeros to kpad),
+// out layout, blocked by 32 k and 64 rows (R = ceil(rows / 64) row blocks, rows padded
+// with zero digits to 64 R, k to kpad, a multiple of 64):
+//   slices[q*S*64R*kpad + (((p/32)*R + r/64)*S + s)*2048 + (r%64)*32 + p%32]  int8,
+// so that one 32-k stage of a 64-row panel (all S slices) is 2048 S contiguous bytes.
 // exps[q*rows + r].  mode kOpN/kOpT/kOpSymL/kOpTriL as rowexp.  exps must be filled by
 // rowexp first.
 hipError_t ozaki_slice(const double* X, long long ld, int mode, int rows, int K, int kpad,
                        int nslices, int nb, long long sx, const int* exps, int8_t* slices,
                        hipStream_t s);
 struct OzakiProb {
-    const int8_t* sa;   // slices of op(A) rows   [S][m][kpad]
-    const int8_t* sb;   // slices of op(B) columns [S][n][kpad]
+    const int8_t* sa;   // slices of op(A) rows   [kpad/32][ceil(m/64)][S][64][32]
+    const int8_t* sb;   // slices of op(B) columns [kpad/32][ceil(n/64)][S][64][32]
     const int8_t* sa2;  // second pair (SYR2K) or null
     const int8_t* sb2;
     const int* ea;      // exponents of op(A) rows
