@@ -623,7 +623,8 @@ def run_serve(args, torch, tensorblas):
     arr = np.cumsum(g.exponential(1.0 / (args.load * cap), count)) - 0.0
     arr -= arr[0]
     res = tensorblas.serve(slots, budget, [dict(r, arrival_us=float(a)) for r, a in zip(reqs, arr)],
-                           large_us=args.large_ms * 1e3, max_large=args.max_large)
+                           large_us=args.large_ms * 1e3, max_large=args.max_large, order=args.order,
+                           reserve_cus=args.reserve_cus)
     lat = [x["done_us"] - a for x, a in zip(res, arr)]
     ratio = [l_ / i_ for l_, i_ in zip(lat, iso)]
     mix_sum = [checksum(torch, C) for _, _, C, _ in keep]
@@ -641,8 +642,10 @@ def run_serve(args, torch, tensorblas):
                      "synthetic: U[-1,1] operands resident in HBM (TRSM: diag n+U[0,1]), seeded",
                      {"workload": f"{count} requests, mix SYRK 40/TRSM 30/TRMM 15/SYMM 10/SYR2K 5, "
                                   f"n log-uniform [128, 8192], k=n, tier uniform over plain/split/ozaki",
-                      "slots": nslots, "hbm_budget_gib": args.budget_gib, "load": args.load,
-                      "large_ms": args.large_ms, "max_large": args.max_large,
+                      "slots": nslots, "hw_queues": args.hw_queues,
+                      "hbm_budget_gib": args.budget_gib, "load": args.load,
+                      "large_ms": args.large_ms, "max_large": args.max_large, "order": args.order,
+                      "reserve_cus": args.reserve_cus,
                       "total_flops": total_flops})
     line.update({
         "latency_ms": {"p50": round(pct(lat, 50) / 1e3, 3), "p99": round(pct(lat, 99) / 1e3, 3)},
@@ -656,6 +659,13 @@ def run_serve(args, torch, tensorblas):
         "refused": sum(1 for x in res if x["status"] == 3),
         "errors": sum(1 for x in res if x["status"] not in (0, 3)),
     })
+    if args.dump:
+        with open(args.dump, "w") as f:
+            json.dump([{"op": r["op"], "tier": r["tier"], "n": r["n"], "flops": r["flops"],
+                        "est_us": x["est_us"], "iso_us": i_, "arrival_us": float(a),
+                        "admit_us": x["admit_us"], "done_us": x["done_us"], "slot": x["slot"],
+                        "offline_done_us": o["done_us"]}
+                       for r, x, i_, a, o in zip(reqs, res, iso, arr, off)], f)
     for h in slots:
         h.close()
     return line
@@ -716,9 +726,14 @@ def main():
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=32.0)
     ap.add_argument("--load", type=float, default=0.7)
-    ap.add_argument("--large-ms", type=float, default=3.0)
+    ap.add_argument("--large-ms", type=float, default=10.0)
     ap.add_argument("--max-large", type=int, default=1)
+    ap.add_argument("--order", default="shortest", choices=["fifo", "shortest"])
+    ap.add_argument("--reserve-cus", type=int, default=0)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="serve: HIP hardware queues of the process (default: one per slot, <= 32)")
     ap.add_argument("--count", type=int, default=0)
+    ap.add_argument("--dump", default="", help="serve: write per-request records (JSON) here")
     ap.add_argument("--tier", default="")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--m", type=int, default=0)
@@ -733,6 +748,12 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1")
+    if args.workload == "serve":
+        # One hardware queue per slot stream, set before HIP starts.  With fewer queues
+        # than slots HIP maps several slot streams onto one queue, and a small request's
+        # kernels then wait in FIFO order behind a large request's on another slot.
+        args.hw_queues = min(32, max(1, args.hw_queues or args.slots))
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
 
     import torch
     import tensorblas

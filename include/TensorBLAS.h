@@ -84,6 +84,12 @@ TB_API int tb_set_ozaki_slices(tb_handle_t handle, int nslices);
  * exact for FP32 inputs, six products (hh | hm+mh | hl+mm+lh) at twice the MFMA work,
  * for inputs whose rows span many binades. */
 TB_API int tb_set_split_planes(tb_handle_t handle, int nplanes);
+/* Occupancy cap (no counterpart in the reference; for callers that run several
+ * handles concurrently, such as tb_serve): the handle's GEMM kernels launch at most
+ * the workgroups that fit on max_cus compute units and loop over the remaining tiles,
+ * so that the rest of the device stays free for other streams.  0 (default): one
+ * workgroup per tile.  Results do not depend on it. */
+TB_API int tb_set_max_cus(tb_handle_t handle, int max_cus);
 /* bytes of device workspace the handle currently holds */
 TB_API long tb_workspace_bytes(tb_handle_t handle);
 /* Per-launch profiling with HIP events on the handle's stream (off by default;
@@ -210,7 +216,13 @@ typedef struct {
     long hbm_budget;       /* bytes admitted requests may hold together */
     double large_us;       /* estimated time alone above which a request is large (0: one lane) */
     int max_large;         /* large requests running at once (>= 1) */
+    int order;             /* within a lane: TB_SERVE_FIFO (oldest waiting request first) or
+                              TB_SERVE_SHORTEST (shortest estimated time alone first) */
+    int reserve_cus;       /* compute units large requests leave free (their GEMMs run
+                              under tb_set_max_cus), so that small requests always find
+                              free CUs; 0: none */
 } tb_serve_config;
+enum { TB_SERVE_FIFO = 0, TB_SERVE_SHORTEST = 1 };
 TB_API int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const tb_request* reqs,
              int count, tb_request_result* results);
 /* Grow a handle's two device workspaces (routines; Ozaki products) to at least these

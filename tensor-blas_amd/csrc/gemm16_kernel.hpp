@@ -536,33 +536,46 @@ gemm16_kernel(GemmLaunch L)
 {
     using CFG = Gemm16Cfg<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>;
     __shared__ Gemm16Lds<CFG, BM, BN> lds;
-    const int g = xcd_remap(blockIdx.x, L.totalTiles);
-    int pi = 0;
+    // workgroup b: tiles b, b + gridDim.x, ... (a capped grid is a multiple of 8, so
+    // xcd_remap keeps every workgroup on its XCD's tile range).  Wide tiles (BM = 256)
+    // are never capped and run one tile per workgroup: their registers are full.
+    constexpr bool kLoop = BM < 256;
+    auto tile = [&](int b) {
+        const int g = xcd_remap(b, L.totalTiles);
+        int pi = 0;
 #pragma unroll 1
-    for (int q = 1; q < L.np; ++q)
-        if (g >= L.p[q].tileBegin) pi = q;
-    const GemmProb& P = L.p[pi];
+        for (int q = 1; q < L.np; ++q)
+            if (g >= L.p[q].tileBegin) pi = q;
+        const GemmProb& P = L.p[pi];
 
-    int local = g - P.tileBegin;
-    const int bi = local / P.tilesPer;
-    local -= bi * P.tilesPer;
-    int ti, tj;
-    if (P.modeC == kCLower) {
-        ti = (int)((__builtin_sqrtf(8.0f * (float)local + 1.0f) - 1.0f) * 0.5f);
-        while ((ti + 1) * (ti + 2) / 2 <= local) ++ti;
-        while (ti * (ti + 1) / 2 > local) --ti;
-        tj = local - ti * (ti + 1) / 2;
+        int local = g - P.tileBegin;
+        const int bi = local / P.tilesPer;
+        local -= bi * P.tilesPer;
+        int ti, tj;
+        if (P.modeC == kCLower) {
+            ti = (int)((__builtin_sqrtf(8.0f * (float)local + 1.0f) - 1.0f) * 0.5f);
+            while ((ti + 1) * (ti + 2) / 2 <= local) ++ti;
+            while (ti * (ti + 1) / 2 > local) --ti;
+            tj = local - ti * (ti + 1) / 2;
+        } else {
+            const int per_group = kGroupM * P.tilesN;
+            const int grp = local / per_group;
+            const int first = grp * kGroupM;
+            const int gsz = min(P.tilesM - first, kGroupM);
+            const int in = local - grp * per_group;
+            ti = first + in % gsz;
+            tj = in / gsz;
+        }
+        // ends with a barrier: the next tile may reuse the LDS ring
+        gemm16_tile<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>(P, bi, ti * BM, tj * BN, lds.ring,
+                                                          lds.scA, lds.scB, lds.emax);
+    };
+    if constexpr (kLoop) {
+#pragma unroll 1
+        for (int b = blockIdx.x; b < L.totalTiles; b += gridDim.x) tile(b);
     } else {
-        const int per_group = kGroupM * P.tilesN;
-        const int grp = local / per_group;
-        const int first = grp * kGroupM;
-        const int gsz = min(P.tilesM - first, kGroupM);
-        const int in = local - grp * per_group;
-        ti = first + in % gsz;
-        tj = in / gsz;
+        tile(blockIdx.x);
     }
-    gemm16_tile<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>(P, bi, ti * BM, tj * BN, lds.ring, lds.scA,
-                                                      lds.scB, lds.emax);
 }
 
 }  // namespace tb

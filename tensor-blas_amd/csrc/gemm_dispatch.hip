@@ -8,48 +8,61 @@ namespace tb {
 namespace {
 
 template <int TIER, int BM, int BN, int BK, int WM, int WN>
-hipError_t run(const GemmLaunch& L, hipStream_t s)
+hipError_t run(const GemmLaunch& L, hipStream_t s, int max_cus)
 {
-    hipLaunchKernelGGL((gemm_grouped_kernel<TIER, BM, BN, BK, WM, WN>), dim3(L.totalTiles),
+    auto k = gemm_grouped_kernel<TIER, BM, BN, BK, WM, WN>;
+    hipLaunchKernelGGL(k, dim3(capped_grid(reinterpret_cast<const void*>(k), kThreads, L.totalTiles, max_cus)),
                        dim3(kThreads), 0, s, L);
     return hipGetLastError();
 }
 
 template <int NPL, int SA, int SB, int BM, int BN, int WM, int WN, int NST>
-hipError_t run16(const GemmLaunch& L, hipStream_t s)
+hipError_t run16(const GemmLaunch& L, hipStream_t s, int max_cus)
 {
-    hipLaunchKernelGGL((gemm16_kernel<NPL, SA, SB, BM, BN, WM, WN, NST>), dim3(L.totalTiles),
+    auto k = gemm16_kernel<NPL, SA, SB, BM, BN, WM, WN, NST>;
+    hipLaunchKernelGGL(k, dim3(capped_grid(reinterpret_cast<const void*>(k), 64 * WM * WN, L.totalTiles, max_cus)),
                        dim3(64 * WM * WN), 0, s, L);
     return hipGetLastError();
 }
 
 template <int NPL, int SA, int SB>
-hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s)
+hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s, int mc)
 {
     // four waves; a stage is (BM + BN) x 64 B per plane, and the ring holds as many
     // stages (<= 4) as fit in the 160 KB of LDS.  Square and leaf tiles: 64 x 64
     // accumulators per wave.  Wide tiles: 128 x 64 per wave, which reads 25 % fewer LDS
     // bytes per MFMA; at 2 planes its two accumulator sets fill the 256 AGPRs.
     if (shape == kShapeLeaf)
-        return run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s);
+        return run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s, mc);
     if constexpr (NPL <= 2 && SA == kSrcPlanes && SB == kSrcPlanes) {
-        if (shape == kShapeWide) return run16<NPL, SA, SB, 256, 128, 2, 2, NPL == 2 ? 3 : 4>(L, s);
+        if (shape == kShapeWide) return run16<NPL, SA, SB, 256, 128, 2, 2, NPL == 2 ? 3 : 4>(L, s, 0);
     }
     if (shape == kShapeWide) return hipErrorInvalidValue;
-    return run16<NPL, SA, SB, 128, 128, 2, 2, NPL == 3 ? 3 : 4>(L, s);
+    return run16<NPL, SA, SB, 128, 128, 2, 2, NPL == 3 ? 3 : 4>(L, s, mc);
 }
 
 template <int NPL>
-hipError_t run16_src(int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s)
+hipError_t run16_src(int shape, int srcA, int srcB, const GemmLaunch& L, hipStream_t s, int mc)
 {
-    if (srcA == kSrcF32 && srcB == kSrcF32) return run16_shape<NPL, kSrcF32, kSrcF32>(shape, L, s);
-    if (srcA == kSrcF32 && srcB == kSrcPlanes) return run16_shape<NPL, kSrcF32, kSrcPlanes>(shape, L, s);
+    if (srcA == kSrcF32 && srcB == kSrcF32) return run16_shape<NPL, kSrcF32, kSrcF32>(shape, L, s, mc);
+    if (srcA == kSrcF32 && srcB == kSrcPlanes) return run16_shape<NPL, kSrcF32, kSrcPlanes>(shape, L, s, mc);
     if (srcA == kSrcPlanes && srcB == kSrcPlanes)
-        return run16_shape<NPL, kSrcPlanes, kSrcPlanes>(shape, L, s);
+        return run16_shape<NPL, kSrcPlanes, kSrcPlanes>(shape, L, s, mc);
     return hipErrorInvalidValue;
 }
 
 }  // namespace
+
+int capped_grid(const void* kernel, int threads, int total, int max_cus)
+{
+    if (max_cus <= 0 || total <= 8) return total;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1)
+        per = 1;
+    long long cap = (long long)max_cus * per / 8 * 8;
+    if (cap < 8) cap = 8;
+    return cap < total ? (int)cap : total;
+}
 
 TileShape gemm_tile_shape(int tier, int shape)
 {
@@ -64,23 +77,23 @@ int tier_planes(int tier, int split_planes)
 }
 
 hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const GemmLaunch& L,
-                       hipStream_t s)
+                       hipStream_t s, int max_cus)
 {
     if (L.totalTiles <= 0) return hipSuccess;
     switch (tier) {
-    case kPlainF16: return run16_src<1>(shape, srcA, srcB, L, s);
+    case kPlainF16: return run16_src<1>(shape, srcA, srcB, L, s, max_cus);
     case kSplitF16:
-        if (npl == 2) return run16_src<2>(shape, srcA, srcB, L, s);
-        if (npl == 3) return run16_src<3>(shape, srcA, srcB, L, s);
+        if (npl == 2) return run16_src<2>(shape, srcA, srcB, L, s, max_cus);
+        if (npl == 3) return run16_src<3>(shape, srcA, srcB, L, s, max_cus);
         return hipErrorInvalidValue;
     case kFmaF32:
         if (srcA != kSrcF32 || srcB != kSrcF32) return hipErrorInvalidValue;
-        return shape == kShapeLeaf ? run<kFmaF32, 64, 256, 32, 1, 4>(L, s)
-                                   : run<kFmaF32, 128, 128, 32, 2, 2>(L, s);
+        return shape == kShapeLeaf ? run<kFmaF32, 64, 256, 32, 1, 4>(L, s, max_cus)
+                                   : run<kFmaF32, 128, 128, 32, 2, 2>(L, s, max_cus);
     case kMfmaF64:
         if (srcA != kSrcF32 || srcB != kSrcF32) return hipErrorInvalidValue;
-        return shape == kShapeLeaf ? run<kMfmaF64, 64, 256, 16, 1, 4>(L, s)
-                                   : run<kMfmaF64, 128, 128, 16, 2, 2>(L, s);
+        return shape == kShapeLeaf ? run<kMfmaF64, 64, 256, 16, 1, 4>(L, s, max_cus)
+                                   : run<kMfmaF64, 128, 128, 16, 2, 2>(L, s, max_cus);
     default:
         return hipErrorInvalidValue;
     }

@@ -240,9 +240,10 @@ struct GemmCfg {
 };
 
 // ---------------------------------------------------------------- the kernel
+// One output tile (flat index g over the launch) per call, computed by the whole
+// workgroup.
 template <int TIER, int BM, int BN, int BK, int WM, int WN>
-__global__ void __launch_bounds__(kThreads)
-gemm_grouped_kernel(GemmLaunch L)
+__device__ __forceinline__ void gemm_grouped_tile(const GemmLaunch& L, int g)
 {
     using Cfg = GemmCfg<TIER, BM, BN, BK, WM, WN>;
     using TT = TierTraits<TIER>;
@@ -256,7 +257,6 @@ gemm_grouped_kernel(GemmLaunch L)
     __shared__ __attribute__((aligned(16))) Lds smem[2 * Cfg::STAGE];
 
     const int tid = threadIdx.x;
-    const int g = xcd_remap(blockIdx.x, L.totalTiles);
     int pi = 0;
 #pragma unroll 1
     for (int q = 1; q < L.np; ++q)
@@ -532,6 +532,20 @@ gemm_grouped_kernel(GemmLaunch L)
         for (int r = tid; r < BM; r += kThreads)
             if (emax_s[r] != kExpZeroDev && i0 + r < M)
                 atomicMax(P.eOut + (long long)bi * P.seA + i0 + r, emax_s[r]);
+    }
+}
+
+// Workgroup b computes tiles b, b + gridDim.x, ... (one tile each when the grid covers
+// every tile; a capped grid, a multiple of 8, keeps each workgroup on its XCD's tile
+// range through xcd_remap).
+template <int TIER, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(kThreads)
+gemm_grouped_kernel(GemmLaunch L)
+{
+#pragma unroll 1
+    for (int b = blockIdx.x; b < L.totalTiles; b += gridDim.x) {
+        gemm_grouped_tile<TIER, BM, BN, BK, WM, WN>(L, xcd_remap(b, L.totalTiles));
+        __syncthreads();      // the next tile reuses the LDS stages
     }
 }
 

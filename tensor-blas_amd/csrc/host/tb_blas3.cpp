@@ -131,7 +131,8 @@ int GemmQueue::add(const GemmProb& p0)
 bool GemmQueue::retile_wide()
 {
     const int npl = planes_of(h_, tier_);
-    if (shape_ != kShapeSquare || srcA_ != kSrcPlanes || srcB_ != kSrcPlanes || npl < 1 || npl > 2)
+    if (shape_ != kShapeSquare || srcA_ != kSrcPlanes || srcB_ != kSrcPlanes || npl < 1 || npl > 2 ||
+        h_->max_cus > 0)          // wide tiles launch one workgroup per tile: never capped
         return false;
     const TileShape w = gemm_tile_shape(tier_, kShapeWide);
     long long tiles = 0;
@@ -162,7 +163,8 @@ int GemmQueue::flush()
     const int shape = retile_wide() ? kShapeWide : shape_;
     hipEvent_t e0;
     TB_TRY(prof_begin(h_, &e0));
-    hipError_t e = gemm_launch(tier_, planes_of(h_, tier_), shape, srcA_, srcB_, L_, h_->stream);
+    hipError_t e = gemm_launch(tier_, planes_of(h_, tier_), shape, srcA_, srcB_, L_, h_->stream,
+                               h_->max_cus);
     std::memset(&L_, 0, sizeof(L_));
     TB_HIP(e);
     return prof_end(h_, kProfGemm0 + tier_ * 2 + shape_, e0, flops);

@@ -88,6 +88,13 @@ int tb_set_ozaki_slices(tb_handle_t h, int nslices)
     return TB_SUCCESS;
 }
 
+int tb_set_max_cus(tb_handle_t h, int max_cus)
+{
+    if (!h || max_cus < 0) return TB_ERR_INVALID;
+    h->max_cus = max_cus;
+    return TB_SUCCESS;
+}
+
 int tb_set_split_planes(tb_handle_t h, int nplanes)
 {
     if (!h || nplanes < 2 || nplanes > 3) return TB_ERR_INVALID;

@@ -24,6 +24,7 @@ struct tb_handle_s {
     int device = 0;
     int ozaki_slices = 7;
     int split_planes = 2;   // FP16 planes per operand of the split tier (2 or 3)
+    int max_cus = 0;        // > 0: GEMM launches occupy at most this many CUs
     // device workspaces, grown on demand, freed by tb_destroy: slot 0 for routines,
     // slot 1 for the Ozaki GEMM (which routines call with their own slot live)
     void* ws[2] = {nullptr, nullptr};

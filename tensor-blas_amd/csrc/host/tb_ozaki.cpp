@@ -84,7 +84,7 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     const double flops = (lowerC ? (double)n * (n + 1) * k : 2.0 * m * n * k) * (pairs ? 2 : 1);
     hipEvent_t e0;
     TB_TRY(prof_begin(h, &e0));
-    TB_HIP(ozaki_gemm(p, st));
+    TB_HIP(ozaki_gemm(p, st, h->max_cus));
     return prof_end(h, kProfOzakiGemm, e0, flops);
 }
 

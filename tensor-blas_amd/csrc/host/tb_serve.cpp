@@ -4,13 +4,21 @@
 // completions are detected by polling events, and each request's times come from HIP
 // events against one start event, so they are device times.
 //
-// Admission: two first-come first-served lanes.  A request whose estimated time alone
-// (flops / per-tier rate, below) exceeds cfg->large_us is large; at most
-// cfg->max_large large requests run at once, so the longest requests do not share the
-// device with each other, and small requests never queue behind a large one.  Within
-// a lane the oldest waiting request is admitted as soon as a slot is free and the
-// bytes it holds fit the budget; later arrivals in its lane never overtake it.  Work
-// submitted on one slot stream never waits for another slot.
+// Admission: two lanes.  A request whose estimated time alone (a per-routine, per-tier
+// model, below) exceeds cfg->large_us is large; at most cfg->max_large large requests
+// run at once, so the longest requests do not share the device with each other, and
+// small requests never queue behind a large one.  Within a lane the next request --
+// the oldest waiting one (TB_SERVE_FIFO) or the one with the shortest estimate
+// (TB_SERVE_SHORTEST) -- is admitted as soon as a slot is free and the bytes it holds
+// fit the budget; no other request of its lane overtakes it.  Work submitted on one
+// slot stream never waits for another slot.
+//
+// A kernel whose workgroups cannot all be resident keeps the CUs it is given as they
+// free up, so kernels from other streams start only once it has dispatched its last
+// workgroup: a small request admitted next to a large one would wait for most of it.
+// With cfg->reserve_cus > 0 the GEMMs of large requests are capped (tb_set_max_cus) at
+// (CUs - reserve_cus) / max_large CUs; their workgroups then all fit and loop over the
+// tiles, and the reserved CUs stay free for the small lane.
 #include <algorithm>
 #include <chrono>
 #include <thread>
@@ -44,9 +52,7 @@ double operand_elems(const tb_request& r)
 // bounded above by twice the operands.
 size_t default_bytes(const tb_request& r) { return (size_t)(2.0 * operand_elems(r) * elem_bytes(r.tier)); }
 
-// Algorithmic flops of a request and its estimated time alone: a fixed launch cost
-// plus flops at a per-tier rate (MI355X, large sizes; small sizes run below these
-// rates, which the fixed term covers in part).
+// Algorithmic flops of a request and its estimated time alone.
 double req_flops(const tb_request& r)
 {
     const double m = (double)r.m, n = (double)r.n, k = (double)r.k;
@@ -60,11 +66,30 @@ double req_flops(const tb_request& r)
     }
 }
 
+// Time alone on MI355X, t = a + b n + flops / c, per routine and tier (plain, split,
+// Ozaki): a covers launches and conversions, b the dependent chain of a recursion
+// (TRSM's n/nb leaves), c the rate of its products.  Fitted to isolated calls of
+// every routine and tier at n = 128 .. 8192 (k = n, m = n); median error 5 %.
+// The FP32 and FP64 comparator tiers use a fixed cost and a flat rate.
+struct TimeModel {
+    double a_us, b_us_per_n, c_tflops;
+};
 double est_us(const tb_request& r)
 {
-    static const double tflops[5] = {250.0, 150.0, 28.0, 70.0, 40.0};   // by tier
-    const int t = r.tier >= 0 && r.tier < 5 ? r.tier : 0;
-    return 50.0 + req_flops(r) / (tflops[t] * 1e6);
+    static const TimeModel model[5][3] = {
+        /* SYRK  */ {{32.0, 0.007, 640.0}, {137.0, 0.015, 269.0}, {174.0, 0.034, 46.0}},
+        /* SYR2K */ {{49.0, 0.002, 613.0}, {272.0, 0.013, 248.0}, {343.0, 0.0, 41.5}},
+        /* TRMM  */ {{37.0, 0.015, 383.0}, {153.0, 0.014, 143.0}, {146.0, 0.031, 25.2}},
+        /* TRSM  */ {{620.0, 0.513, 242.0}, {597.0, 0.708, 191.0}, {43.5, 1.351, 45.7}},
+        /* SYMM  */ {{32.0, 0.024, 715.0}, {149.0, 0.021, 268.0}, {137.0, 0.044, 48.1}},
+    };
+    const double f = req_flops(r);
+    if (r.tier == TB_TIER_FP32) return 50.0 + f / (70.0 * 1e6);
+    if (r.tier == TB_TIER_FP64) return 50.0 + f / (40.0 * 1e6);
+    const int t = r.tier == TB_TIER_PLAIN ? 0 : (r.tier == TB_TIER_SPLIT ? 1 : 2);
+    const TimeModel& m = model[r.op][t];
+    const double n = (double)(r.op == TB_REQ_TRMM || r.op == TB_REQ_SYMM ? r.m : r.n);
+    return m.a_us + m.b_us_per_n * n + f / (m.c_tflops * 1e6);
 }
 
 int launch(tb_handle_t h, const tb_request& r)
@@ -115,8 +140,18 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
              int count, tb_request_result* out)
 {
     if (!slots || nslots <= 0 || !cfg || cfg->hbm_budget <= 0 || cfg->large_us < 0 ||
-        cfg->max_large < 1 || count < 0 || (count > 0 && (!reqs || !out)))
+        cfg->max_large < 1 || (cfg->order != TB_SERVE_FIFO && cfg->order != TB_SERVE_SHORTEST) ||
+        cfg->reserve_cus < 0 || count < 0 || (count > 0 && (!reqs || !out)))
         return TB_ERR_INVALID;
+    int large_cus = 0;                  // CU cap of a large request's GEMMs (0: none)
+    if (cfg->reserve_cus > 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, slots[0]->device) !=
+            hipSuccess)
+            return TB_ERR_DEVICE;
+        if (cfg->reserve_cus >= cus) return TB_ERR_INVALID;
+        large_cus = std::max(1, (cus - cfg->reserve_cus) / cfg->max_large);
+    }
     const size_t budget = (size_t)cfg->hbm_budget;
     for (int s = 0; s < nslots; ++s)
         if (!slots[s]) return TB_ERR_INVALID;
@@ -161,7 +196,8 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
     std::vector<Running> run;
     size_t used = 0;
     int large_running = 0, finished = 0;
-    size_t next[2] = {0, 0};
+    size_t next[2] = {0, 0};           // next request of each lane still to arrive
+    std::vector<int> waiting[2];       // arrived, not admitted, in arrival order
     while (st == TB_SUCCESS && finished < count) {
         bool progress = false;
         // completions
@@ -192,26 +228,37 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
             while (next[L] < lane[L].size()) {
                 const int r = lane[L][next[L]];
                 if (reqs[r].arrival_us > now_us) break;
+                ++next[L];
+                progress = true;
                 if (need[r] > budget) {                       // can never fit: refuse it
                     out[r].status = TB_ERR_NOMEM;
                     out[r].admit_us = now_us;
-                    ++next[L];
                     ++finished;
-                    progress = true;
                     continue;
                 }
+                waiting[L].push_back(r);
+            }
+            while (!waiting[L].empty()) {
+                size_t w = 0;
+                if (cfg->order == TB_SERVE_SHORTEST)
+                    for (size_t q = 1; q < waiting[L].size(); ++q)
+                        if (out[waiting[L][q]].est_us < out[waiting[L][w]].est_us) w = q;
+                const int r = waiting[L][w];
                 if (idle.empty() || used + need[r] > budget) break;
                 if (L == 1 && large_running >= cfg->max_large) break;
                 const int s = idle.back();
                 idle.pop_back();
                 out[r].admit_us = now_us;
                 out[r].slot = s;
+                const int keep_cus = slots[s]->max_cus;
+                if (L == 1 && large_cus > 0) slots[s]->max_cus = large_cus;
                 out[r].status = launch(slots[s], reqs[r]);
+                slots[s]->max_cus = keep_cus;
                 if (hipEventRecord(done[s], slots[s]->stream) != hipSuccess) st = TB_ERR_DEVICE;
                 used += need[r];
                 large_running += L;
                 run.push_back({r, s, need[r], L == 1});
-                ++next[L];
+                waiting[L].erase(waiting[L].begin() + (long)w);
                 progress = true;
             }
         }

@@ -174,139 +174,146 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
     static_assert(DMA_W == S && KEEP <= 60, "vmcnt");
     __shared__ __attribute__((aligned(1024))) int8_t smem[NST * STAGE];
 
-    const int g = xcd_remap(blockIdx.x, totalTiles);
-    int ti, tj;
-    if (P.lowerC) {
-        lower_tile_banded(g, tilesM, ti, tj);
-    } else {
-        // groups of kBand tile rows, column by column inside a group
-        const int tilesN = totalTiles / tilesM;
-        const int per = kBand * tilesN;
-        const int first = (g / per) * kBand;
-        const int gsz = min(tilesM - first, kBand);
-        const int in = g - (g / per) * per;
-        ti = first + in % gsz;
-        tj = in / gsz;
-    }
-    const int i0 = ti * OBM, j0 = tj * OBN;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave & 1, wn = wave >> 1;
-
-    // this wave's direct-to-LDS loads: instruction q moves 32 rows (1 KB) of one slice of
-    // one panel; 4*S instructions per stage, S per wave.  In the panel-blocked slice
-    // layout a panel's stage (S slices x 64 rows x 32 k) is one contiguous block laid
-    // out as in LDS, and stage kb of a panel is kb * (row blocks) * S * PLANE further.
-    const int8_t* gsrc[DMA_W];
-    bool isB[DMA_W];
-    unsigned ldo[DMA_W];
-    const long long kstepA = (long long)((P.m + OBM - 1) / OBM) * S * PLANE;
-    const long long kstepB = (long long)((P.n + OBN - 1) / OBN) * S * PLANE;
-    int pair_src = -1;
-    const unsigned smem_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)smem;
-    auto dma_setup = [&](int pair) {
-        const int8_t* GA = pair ? P.sa2 : P.sa;
-        const int8_t* GB = pair ? P.sb2 : P.sb;
-#pragma unroll
-        for (int q = 0; q < DMA_W; ++q) {
-            const int inst = wave + q * 4;                  // 0 .. 4S-1
-            const int panel = inst / (2 * S);               // 0: A, 1: B
-            const int rest = inst % (2 * S);
-            const int sl = rest / 2, blk = rest % 2;
-            const int rl = blk * 32 + (lane >> 1);          // LDS row written by this lane
-            const int u = (lane & 1) ^ ((rl >> 3) & 1);     // logical unit stored there
-            const int tile = panel ? tj : ti;               // 64-row block of the operand
-            gsrc[q] = (panel ? GB : GA) + ((long long)tile * S + sl) * PLANE + rl * OBK + u * 16;
-            isB[q] = panel != 0;
-            ldo[q] = (unsigned)((panel * S + sl) * PLANE + blk * 1024);
-        }
-    };
-    const int npairs = P.sa2 ? 2 : 1;
-    const int kblocks = P.kpad / OBK;
-    const int nblk = npairs * kblocks;
-    auto issue = [&](int b) {
-        const int pair = b >= kblocks;
-        if (pair != pair_src) {
-            dma_setup(pair);
-            pair_src = pair;
-        }
-        const int kb = b - pair * kblocks;
-        const unsigned base = smem_lds + (unsigned)((b % NST) * STAGE);
-#pragma unroll
-        for (int q = 0; q < DMA_W; ++q) oz_dma16(gsrc[q] + kb * (isB[q] ? kstepB : kstepA), base + ldo[q]);
-    };
-
-    intx16_t acc[S];
-    double facc[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) facc[r] = 0.0;
-#pragma unroll
-    for (int q = 0; q < S; ++q) acc[q] = intx16_t{};
-
-    const int cblocks = P.kchunk / OBK;          // k blocks per exactness chunk
-    // fragment of lane: row (panel base + lane%32), logical unit lane/32
-    const int fr = lane & 31;
-    const int fo = fr * OBK + (((lane >> 5) ^ ((fr >> 3) & 1)) * 16);
-    const int offA = wm * 32 * OBK + fo, offB = S * PLANE + wn * 32 * OBK + fo;
-
-    if (nblk > 0) {
-        for (int b = 0; b < NST - 1; ++b)
-            if (b < nblk) issue(b);
-        // stage 0 complete: only the other issued stages (DMA_W loads each) may be in flight
-        const int issued = nblk < NST - 1 ? nblk : NST - 1;
-        if (issued == 3) TB_OZ_WAIT_BARRIER(2 * DMA_W);
-        else if (issued == 2) TB_OZ_WAIT_BARRIER(DMA_W);
-        else TB_OZ_WAIT_BARRIER(0);
-    }
+    // workgroup blk: tiles blk, blk + gridDim.x, ... (a capped grid is a multiple of 8,
+    // so xcd_remap keeps every workgroup on its XCD's tile range).  Each tile's k loop
+    // ends with a barrier after the last LDS reads, so the next tile may refill the ring.
 #pragma unroll 1
-    for (int b = 0; b < nblk; ++b) {
-        const int8_t* st = smem + (b % NST) * STAGE;
-        intx4_t af[S], bf[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            af[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offA);
-            bf[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offB);
+    for (int blk = blockIdx.x; blk < totalTiles; blk += gridDim.x) {
+        const int g = xcd_remap(blk, totalTiles);
+        int ti, tj;
+        if (P.lowerC) {
+            lower_tile_banded(g, tilesM, ti, tj);
+        } else {
+            // groups of kBand tile rows, column by column inside a group
+            const int tilesN = totalTiles / tilesM;
+            const int per = kBand * tilesN;
+            const int first = (g / per) * kBand;
+            const int gsz = min(tilesM - first, kBand);
+            const int in = g - (g / per) * per;
+            ti = first + in % gsz;
+            tj = in / gsz;
         }
-        // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
-        const bool ahead = b + NST - 1 < nblk;
-        if (ahead) issue(b + NST - 1);
-        // D_g += A_s B_{g-s}: consecutive MFMAs update different D_g
+        const int i0 = ti * OBM, j0 = tj * OBN;
+        const int tid = threadIdx.x, lane = tid & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int wm = wave & 1, wn = wave >> 1;
+
+        // this wave's direct-to-LDS loads: instruction q moves 32 rows (1 KB) of one slice of
+        // one panel; 4*S instructions per stage, S per wave.  In the panel-blocked slice
+        // layout a panel's stage (S slices x 64 rows x 32 k) is one contiguous block laid
+        // out as in LDS, and stage kb of a panel is kb * (row blocks) * S * PLANE further.
+        const int8_t* gsrc[DMA_W];
+        bool isB[DMA_W];
+        unsigned ldo[DMA_W];
+        const long long kstepA = (long long)((P.m + OBM - 1) / OBM) * S * PLANE;
+        const long long kstepB = (long long)((P.n + OBN - 1) / OBN) * S * PLANE;
+        int pair_src = -1;
+        const unsigned smem_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)smem;
+        auto dma_setup = [&](int pair) {
+            const int8_t* GA = pair ? P.sa2 : P.sa;
+            const int8_t* GB = pair ? P.sb2 : P.sb;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
+            for (int q = 0; q < DMA_W; ++q) {
+                const int inst = wave + q * 4;                  // 0 .. 4S-1
+                const int panel = inst / (2 * S);               // 0: A, 1: B
+                const int rest = inst % (2 * S);
+                const int sl = rest / 2, blk = rest % 2;
+                const int rl = blk * 32 + (lane >> 1);          // LDS row written by this lane
+                const int u = (lane & 1) ^ ((rl >> 3) & 1);     // logical unit stored there
+                const int tile = panel ? tj : ti;               // 64-row block of the operand
+                gsrc[q] = (panel ? GB : GA) + ((long long)tile * S + sl) * PLANE + rl * OBK + u * 16;
+                isB[q] = panel != 0;
+                ldo[q] = (unsigned)((panel * S + sl) * PLANE + blk * 1024);
+            }
+        };
+        const int npairs = P.sa2 ? 2 : 1;
+        const int kblocks = P.kpad / OBK;
+        const int nblk = npairs * kblocks;
+        auto issue = [&](int b) {
+            const int pair = b >= kblocks;
+            if (pair != pair_src) {
+                dma_setup(pair);
+                pair_src = pair;
+            }
+            const int kb = b - pair * kblocks;
+            const unsigned base = smem_lds + (unsigned)((b % NST) * STAGE);
 #pragma unroll
-            for (int gg = s; gg < S; ++gg)
-                acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
+            for (int q = 0; q < DMA_W; ++q) oz_dma16(gsrc[q] + kb * (isB[q] ? kstepB : kstepA), base + ldo[q]);
+        };
+
+        intx16_t acc[S];
+        double facc[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) facc[r] = 0.0;
+#pragma unroll
+        for (int q = 0; q < S; ++q) acc[q] = intx16_t{};
+
+        const int cblocks = P.kchunk / OBK;          // k blocks per exactness chunk
+        // fragment of lane: row (panel base + lane%32), logical unit lane/32
+        const int fr = lane & 31;
+        const int fo = fr * OBK + (((lane >> 5) ^ ((fr >> 3) & 1)) * 16);
+        const int offA = wm * 32 * OBK + fo, offB = S * PLANE + wn * 32 * OBK + fo;
+
+        if (nblk > 0) {
+            for (int b = 0; b < NST - 1; ++b)
+                if (b < nblk) issue(b);
+            // stage 0 complete: only the other issued stages (DMA_W loads each) may be in flight
+            const int issued = nblk < NST - 1 ? nblk : NST - 1;
+            if (issued == 3) TB_OZ_WAIT_BARRIER(2 * DMA_W);
+            else if (issued == 2) TB_OZ_WAIT_BARRIER(DMA_W);
+            else TB_OZ_WAIT_BARRIER(0);
         }
-        const int kb = b % kblocks;
-        if ((kb + 1) % cblocks == 0 || kb + 1 == kblocks) {
-            // exactness chunk done: D_g into FP64, increasing g
+#pragma unroll 1
+        for (int b = 0; b < nblk; ++b) {
+            const int8_t* st = smem + (b % NST) * STAGE;
+            intx4_t af[S], bf[S];
 #pragma unroll
-            for (int gg = 0; gg < S; ++gg) {
+            for (int s = 0; s < S; ++s) {
+                af[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offA);
+                bf[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offB);
+            }
+            // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
+            const bool ahead = b + NST - 1 < nblk;
+            if (ahead) issue(b + NST - 1);
+            // D_g += A_s B_{g-s}: consecutive MFMAs update different D_g
 #pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    facc[r] = facc[r] + __builtin_ldexp((double)acc[gg][r], -7 * (gg + 2));
-                acc[gg] = intx16_t{};
+            for (int s = 0; s < S; ++s) {
+#pragma unroll
+                for (int gg = s; gg < S; ++gg)
+                    acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
+            }
+            const int kb = b % kblocks;
+            if ((kb + 1) % cblocks == 0 || kb + 1 == kblocks) {
+                // exactness chunk done: D_g into FP64, increasing g
+#pragma unroll
+                for (int gg = 0; gg < S; ++gg) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        facc[r] = facc[r] + __builtin_ldexp((double)acc[gg][r], -7 * (gg + 2));
+                    acc[gg] = intx16_t{};
+                }
+            }
+            // block b+1 complete in LDS (later stages may still be in flight)
+            if (ahead) TB_OZ_WAIT_BARRIER(KEEP);
+            else TB_OZ_WAIT_BARRIER(0);
+        }
+
+        // epilogue: D[jj][ii], ii = lane%32 runs along C's rows
+        const int i = i0 + wm * 32 + (lane & 31);
+        if (i < P.m) {
+            const int ea = P.ea[i];
+            const int sea = ea < -5000 ? 0 : ea;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = j0 + wn * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                if (j >= P.n || (P.lowerC && i < j)) continue;
+                const int eb = P.eb[j];
+                const int seb = eb < -5000 ? 0 : eb;
+                const double v = __builtin_ldexp(facc[r], sea + seb);
+                double* c = P.C + (long long)i + (long long)j * P.ldc;
+                *c = (P.beta == 0.0) ? P.alpha * v : __builtin_fma(P.alpha, v, P.beta * *c);
             }
         }
-        // block b+1 complete in LDS (later stages may still be in flight)
-        if (ahead) TB_OZ_WAIT_BARRIER(KEEP);
-        else TB_OZ_WAIT_BARRIER(0);
-    }
-
-    // epilogue: D[jj][ii], ii = lane%32 runs along C's rows
-    const int i = i0 + wm * 32 + (lane & 31);
-    if (i >= P.m) return;
-    const int ea = P.ea[i];
-    const int sea = ea < -5000 ? 0 : ea;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int j = j0 + wn * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        if (j >= P.n || (P.lowerC && i < j)) continue;
-        const int eb = P.eb[j];
-        const int seb = eb < -5000 ? 0 : eb;
-        const double v = __builtin_ldexp(facc[r], sea + seb);
-        double* c = P.C + (long long)i + (long long)j * P.ldc;
-        *c = (P.beta == 0.0) ? P.alpha * v : __builtin_fma(P.alpha, v, P.beta * *c);
     }
 }
 
@@ -324,7 +331,7 @@ hipError_t ozaki_slice(const double* X, long long ld, int mode, int rows, int K,
     return hipGetLastError();
 }
 
-hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s)
+hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus)
 {
     if (p.m <= 0 || p.n <= 0) return hipSuccess;
     if (p.nslices < kMinSlices || p.nslices > kMaxSlices || p.kpad % OBK || p.kchunk < OBK ||
@@ -336,7 +343,10 @@ hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s)
     switch (p.nslices) {
 #define TB_OZ_CASE(S_)                                                                            \
     case S_:                                                                                      \
-        hipLaunchKernelGGL(ozaki_gemm_kernel<S_>, dim3(total), dim3(256), 0, s, p, tm, total);    \
+        hipLaunchKernelGGL(ozaki_gemm_kernel<S_>,                                                 \
+                           dim3(capped_grid(reinterpret_cast<const void*>(ozaki_gemm_kernel<S_>), \
+                                            256, total, max_cus)),                                \
+                           dim3(256), 0, s, p, tm, total);                                        \
         break;
         TB_OZ_CASE(3) TB_OZ_CASE(4) TB_OZ_CASE(5) TB_OZ_CASE(6) TB_OZ_CASE(7) TB_OZ_CASE(8)
         TB_OZ_CASE(9) TB_OZ_CASE(10)

@@ -26,9 +26,14 @@ TileShape gemm_tile_shape(int tier, int shape);
 // others 0
 int tier_planes(int tier, int split_planes);
 // all problems of L share the operand source kinds srcA/srcB (GemmProb::srcA/srcB);
-// npl: planes of the split tier (2 or 3)
+// npl: planes of the split tier (2 or 3); max_cus > 0 caps the grid at the workgroups
+// that fit on that many CUs (the kernels loop over the remaining tiles)
 hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const GemmLaunch& L,
-                       hipStream_t s);
+                       hipStream_t s, int max_cus = 0);
+// Grid of a tile-looping kernel: every tile (total) when max_cus <= 0, else at most the
+// workgroups of `kernel` resident on max_cus CUs, rounded down to a multiple of 8 (the
+// XCD count, which the kernels' xcd_remap tile order relies on).
+int capped_grid(const void* kernel, int threads, int total, int max_cus);
 
 // Pre-conversion of the rows of op(X) (FP32, rowexp modes kOpN/kOpT/kOpSymL/kOpTriL,
 // rows x K) into npl K-contiguous FP16 planes: element (r, k) of plane p at
@@ -115,7 +120,7 @@ struct OzakiProb {
     int m, n, kpad, nslices, kchunk;
     int lowerC;
 };
-hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s);
+hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);
 
 // ---------------------------------------------------------------------- batches
 struct RowexpJob;

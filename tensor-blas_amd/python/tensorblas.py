@@ -44,6 +44,7 @@ def lib():
             "tb_create": [_P], "tb_destroy": [_P], "tb_set_stream": [_P, _P],
             "tb_synchronize": [_P], "tb_set_ozaki_slices": [_P, _I],
             "tb_set_split_planes": [_P, _I],
+            "tb_set_max_cus": [_P, _I],
             "tb_syrk": [_P, _I, _L, _L, _D, _P, _L, _D, _P, _L, _L],
             "tb_syr2k": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _L],
             "tb_trmm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
@@ -116,6 +117,11 @@ class Handle:
 
     def set_split_planes(self, n):
         _check("tb_set_split_planes", lib().tb_set_split_planes(self._h, n))
+
+    def set_max_cus(self, n):
+        """Cap the CUs this handle's GEMM kernels occupy (0: no cap); results do not
+        change."""
+        _check("tb_set_max_cus", lib().tb_set_max_cus(self._h, int(n)))
 
     PROF_NAMES = ["gemm_plain_sq", "gemm_plain_leaf", "gemm_split_sq", "gemm_split_leaf",
                   "gemm_ozaki_sq", "gemm_ozaki_leaf", "gemm_fp32_sq", "gemm_fp32_leaf",
@@ -261,18 +267,23 @@ class RequestResult(ctypes.Structure):
 
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_long), ("large_us", ctypes.c_double),
-                ("max_large", ctypes.c_int)]
+                ("max_large", ctypes.c_int), ("order", ctypes.c_int), ("reserve_cus", ctypes.c_int)]
+
+
+SERVE_ORDERS = {"fifo": 0, "shortest": 1}
 
 
 REQ_OPS = {"syrk": 0, "syr2k": 1, "trmm": 2, "trsm": 3, "symm": 4}
 
 
-def serve(slots, budget_bytes, requests, large_us=0.0, max_large=1):
+def serve(slots, budget_bytes, requests, large_us=0.0, max_large=1, order="fifo", reserve_cus=0):
     """Run a request stream (tb_serve) on the given handles; requests are dicts with
     op, tier, m, n, k, nb, alpha, beta, A, lda, B, ldb, C, ldc, arrival_us, bytes.
     large_us > 0 splits admission into a small and a large lane (at most max_large
-    large requests at once).  Returns a list of dicts admit_us, done_us, slot, status,
-    est_us."""
+    large requests at once); order "fifo" admits the oldest waiting request of a lane
+    first, "shortest" the one with the shortest estimated time alone; reserve_cus > 0
+    keeps that many CUs free of large requests' GEMMs.  Returns a list of dicts
+    admit_us, done_us, slot, status, est_us."""
     n = len(requests)
     arr = (Request * max(1, n))()
     for i, r in enumerate(requests):
@@ -282,7 +293,8 @@ def serve(slots, budget_bytes, requests, large_us=0.0, max_large=1):
                          float(r.get("arrival_us", 0.0)), int(r.get("bytes", 0)))
     res = (RequestResult * max(1, n))()
     hs = (ctypes.c_void_p * len(slots))(*[s.ptr.value for s in slots])
-    cfg = ServeConfig(int(budget_bytes), float(large_us), int(max_large))
+    cfg = ServeConfig(int(budget_bytes), float(large_us), int(max_large), SERVE_ORDERS[order],
+                      int(reserve_cus))
     _check("tb_serve", lib().tb_serve(hs, len(slots), ctypes.byref(cfg), arr, n, res))
     return [dict(admit_us=res[i].admit_us, done_us=res[i].done_us, slot=res[i].slot,
                  status=res[i].status, est_us=res[i].est_us) for i in range(n)]

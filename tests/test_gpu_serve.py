@@ -113,9 +113,11 @@ def test_serve_rejects_bad_arguments(tbm):
                                      C=z.data_ptr(), ldc=1, arrival_us=-1.0)])
 
 
-def test_serve_lanes(tbm):
+@pytest.mark.parametrize("reserve_cus", [0, 32])
+def test_serve_lanes(tbm, reserve_cus):
     """Requests estimated above large_us run at most max_large at a time; small ones
-    keep flowing; every result still equals the isolated one."""
+    keep flowing (with reserve_cus, on CUs the large requests' GEMMs leave free); every
+    result still equals the isolated one."""
     reqs, keep = make_requests(9, sizes=(128, 448))
     alone, est = [], []
     h1 = slots(tbm, 1)
@@ -125,7 +127,8 @@ def test_serve_lanes(tbm):
     restore(keep)
     thr = float(np.median(est))                # the library's own estimates split the set
     hs = slots(tbm, 6)
-    res = tbm.serve(hs, 1 << 34, [dict(r, arrival_us=0.0) for r in reqs], large_us=thr, max_large=1)
+    res = tbm.serve(hs, 1 << 34, [dict(r, arrival_us=0.0) for r in reqs], large_us=thr, max_large=1,
+                    reserve_cus=reserve_cus)
     large = [x for x in res if x["est_us"] > thr]
     assert 0 < len(large) < len(res)
     large.sort(key=lambda x: x["admit_us"])
@@ -134,3 +137,23 @@ def test_serve_lanes(tbm):
     for i, x in enumerate(res):
         assert x["status"] == 0
         assert torch.equal(keep[i][2], alone[i]), (reqs[i]["op"], reqs[i]["tier"], reqs[i]["n"])
+
+
+def test_serve_shortest_first(tbm):
+    """With one slot and every request waiting from the start, TB_SERVE_SHORTEST admits
+    in increasing estimated time, FIFO in arrival order; the results do not change."""
+    reqs, keep = make_requests(21, sizes=(64, 384, 160, 256))
+    hs = slots(tbm, 1)
+    mix = [dict(r, arrival_us=0.0) for r in reqs]
+    res = tbm.serve(hs, 1 << 34, mix, order="shortest")
+    assert all(x["status"] == 0 for x in res)
+    by_admit = sorted(range(len(res)), key=lambda i: res[i]["admit_us"])
+    ests = [res[i]["est_us"] for i in by_admit]
+    assert ests == sorted(ests)
+    first = [C.clone() for _, _, C, _ in keep]
+    restore(keep)
+    res = tbm.serve(hs, 1 << 34, mix, order="fifo")
+    by_admit = sorted(range(len(res)), key=lambda i: (res[i]["admit_us"], i))
+    assert by_admit == list(range(len(res)))
+    for i, (_, _, C, _) in enumerate(keep):
+        assert torch.equal(C, first[i]), (reqs[i]["op"], reqs[i]["tier"])

@@ -1,11 +1,12 @@
-"""Tile shapes do not change results.
+"""Tile shapes and launch caps do not change results.
 
 Launches of full-C plane products with at least 256 tiles of 256 x 128 run on those
 wide tiles; smaller launches run on 128 x 128 tiles.  Every output element is summed
 over k in the same order by either shape, so a product computed in one wide launch must
 equal, bit for bit, the same product computed as column blocks small enough to stay on
 square tiles.  The square-tile results are themselves held to the oracle's bounds in
-test_gpu_blas3.py; here the wide result is also held to the same bound.
+test_gpu_blas3.py; here the wide result is also held to the same bound.  Likewise a
+handle whose kernels are capped to a few CUs (tb_set_max_cus) computes the same bits.
 """
 import numpy as np
 import pytest
@@ -101,3 +102,35 @@ def test_trsm_wide_updates_equal_row_blocks(h, tier):
     e = oracle.errors(X[rows], R)
     tol = 2.0 ** -9 if tier == "plain" else 1e-6
     assert e["normwise"] < tol and e["maxrel"] < 4 * tol, e
+
+
+@pytest.mark.parametrize("tier", ["plain", "split", "ozaki", "fp32", "fp64"])
+def test_cu_cap_does_not_change_results(h, tier):
+    """With tb_set_max_cus the GEMM kernels launch fewer workgroups than tiles and loop
+    over them; every tile still sums k in the same order, so SYRK, TRSM and GEMM results
+    equal the uncapped ones bit for bit."""
+    import tensorblas
+    g = rng(77 + len(tier))
+    dt = np.float64 if tier in ("ozaki", "fp64") else np.float32
+    n, k, m = 1500, 300, 700
+    A = uniform(g, (n, k), dtype=dt)
+    L = lower_tri_dominant(g, n, dt)
+    B = uniform(g, (m, n), dtype=dt)
+    outs = []
+    for cap in (0, 24):
+        hc = tensorblas.Handle()
+        hc.set_max_cus(cap)
+        dA, lda = to_dev(A)
+        dC = torch.zeros(n * n, device="cuda", dtype=torch.float64 if dt == np.float64 else torch.float32)
+        hc.syrk(tier, n, k, 1.0, dA.data_ptr(), lda, 0.0, dC.data_ptr(), n)
+        dL, ldl = to_dev(L)
+        dB, ldb = to_dev(B)
+        hc.trsm(tier, m, n, dL.data_ptr(), ldl, dB.data_ptr(), ldb, nb=256)
+        dD = torch.zeros_like(dC)
+        hc.gemm(tier, 0, 1, n, n, k, 1.0, dA.data_ptr(), lda, dA.data_ptr(), lda, 0.0,
+                dD.data_ptr(), n)
+        hc.synchronize()
+        outs.append((dC.cpu(), dB.cpu(), dD.cpu()))
+        hc.close()
+    for a, b, what in zip(outs[0], outs[1], ("syrk", "trsm", "gemm")):
+        assert torch.equal(a, b), f"{tier} {what}: {int((a != b).sum())} entries differ"
