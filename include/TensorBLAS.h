@@ -99,8 +99,9 @@ TB_API long tb_workspace_bytes(tb_handle_t handle);
  * written).  Categories: 0..9 grouped GEMM (tier*2 + tile shape, shape 0 = 128x128
  * updates, 1 = 64x256 TRSM leaves), 10 Ozaki slice GEMM, 11 Ozaki slicing,
  * 12 diagonal-block inversion, 13 FP32 -> FP16 plane conversion of operands, 14 one
- * run of a batch plan (tb_batch_run: prologue and persistent kernel). */
-#define TB_PROF_CATEGORIES 15
+ * run of a batch plan (tb_batch_run: prologue and persistent kernel), 15 POTRF
+ * diagonal-block Cholesky. */
+#define TB_PROF_CATEGORIES 16
 TB_API int tb_set_profiling(tb_handle_t handle, int on);
 TB_API int tb_profile_read(tb_handle_t handle, double* out, int ncat);
 /* human-readable text of a status code */
@@ -132,6 +133,16 @@ TB_API int tb_trsm(tb_handle_t h, int tier, long m, long n, const void* A, long 
  * FP32 tiers only.  No reference counterpart (added by the FP32-accuracy target). */
 TB_API int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* A, long lda, void* B,
                     long ldb, long nb);
+/* Cholesky factorization A = L L^T of a symmetric positive definite n x n matrix, lower
+ * triangle in place (the upper triangle is neither read nor written).  No counterpart
+ * in the reference, which stops at the BLAS-3 routines; this is their canonical
+ * caller, composed from them: recursive halving, A21 := A21 L11^-T by tb_trsm and
+ * A22 -= A21 A21^T by tb_syrk in the given tier, diagonal blocks of at most 128
+ * factored in FP64 on the device.  *info = 0, or j > 0 when the leading minor of order
+ * j is not positive definite (LAPACK dpotrf's info; A then holds a partial factor).
+ * Synchronizes the handle's stream to read info.  float* A for plain/split/fp32,
+ * double* A for ozaki/fp64; nb is passed to tb_trsm and tb_syrk. */
+TB_API int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* info);
 /* replaces tc_symm / tc_cumpsgemm_symm (include/TensorBLAS.h:165-167); the OZAKI and
  * FP64 tiers add the FP64 SYMM the reference does not have */
 TB_API int tb_symm(tb_handle_t h, int tier, long m, long n, double alpha, const void* A, long lda,

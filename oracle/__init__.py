@@ -52,6 +52,8 @@ def lib():
         L.orc_ozaki_gemm.argtypes = [_i64, _i64, _i64, ctypes.c_int, _p, _p, _p, _p, _dbl,
                                      _dbl, _p, _i64, _i64]
         L.orc_errors.argtypes = [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, ctypes.c_int, _p]
+        L.orc_dpotrf_lower.argtypes = [_i64, _p, _i64]
+        L.orc_dpotrf_lower.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -217,3 +219,15 @@ def errors(X, R, S=None, lower=False):
     lib().orc_errors(m, n, _ptr(X), max(1, X.shape[0]), _ptr(R), max(1, R.shape[0]), Sp, lds,
                      int(lower), _ptr(out))
     return {"normwise": out[0], "maxrel": out[1], "componentwise": out[2]}
+
+
+# ------------------------------------------------------------------ Cholesky
+def dpotrf_lower(A):
+    """(L, info): lower Cholesky factor of A (only A's lower triangle is read) in FP64;
+    info = 0, or j+1 when the leading minor of order j+1 is not positive definite (L
+    then holds the partial factorization).  L's strict upper triangle is zero."""
+    L = np.tril(_f64(A)).copy(order="F")
+    n = L.shape[0]
+    info = lib().orc_dpotrf_lower(n, _ptr(L), max(1, n))
+    return L, int(info)
+

@@ -22,6 +22,10 @@
  *      FP64 (BLAS/tc_ozimmu_syrk.cu:33-99); parity for this tier is pinned by exact
  *      integer arithmetic and by the FP64 routines above.
  *   6. error metrics (normwise as testing/testing_syrk.cu:152, max-relative, componentwise)
+ *   7. Cholesky factorization A = L L^T (lower), the downstream caller composed from
+ *      SYRK and TRSM.  No reference counterpart (the reference stops at the BLAS-3
+ *      routines); LAPACK dpotrf semantics, pinned by tests against numpy.linalg.cholesky
+ *      and by the reconstruction L L^T = A.
  *
  * All matrices are column-major, leading dimensions in elements.
  */
@@ -419,4 +423,27 @@ void orc_errors(int64_t m, int64_t n, const double *X, int64_t ldx, const double
     out[0] = den > 0 ? sqrt(num / den) : sqrt(num);
     out[1] = mxr > 0 ? mxd / mxr : mxd;
     out[2] = comp;
+}
+
+/* ------------------------------------------------------------- 7. Cholesky */
+
+/* Lower Cholesky in place, right-looking, unblocked: for j = 0..n-1,
+ * l_jj = sqrt(a_jj), l_ij = a_ij / l_jj (i > j), a_ik -= l_ij l_kj (j < k <= i).
+ * Only the lower triangle is read or written.  Returns 0, or j+1 when the leading
+ * minor of order j+1 is not positive definite (a_jj <= 0 or NaN at step j; A then
+ * holds the partial factorization), as LAPACK dpotrf's info. */
+int orc_dpotrf_lower(int64_t n, double *A, int64_t lda)
+{
+    for (int64_t j = 0; j < n; j++) {
+        double d = A[j + j * lda];
+        if (!(d > 0.0)) return (int)(j + 1);
+        d = sqrt(d);
+        A[j + j * lda] = d;
+        for (int64_t i = j + 1; i < n; i++) A[i + j * lda] /= d;
+        for (int64_t k = j + 1; k < n; k++) {
+            const double lkj = A[k + j * lda];
+            for (int64_t i = k; i < n; i++) A[i + k * lda] -= A[i + j * lda] * lkj;
+        }
+    }
+    return 0;
 }

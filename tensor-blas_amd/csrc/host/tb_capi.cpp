@@ -64,6 +64,7 @@ int tb_destroy(tb_handle_t h)
     prof_clear(h);
     for (int s = 0; s < 2; ++s)
         if (h->ws[s]) (void)hipFree(h->ws[s]);
+    if (h->dev_info) (void)hipFree(h->dev_info);
     delete h;
     return st;
 }
@@ -139,6 +140,11 @@ int tb_trsm(tb_handle_t h, int tier, long m, long n, const void* A, long lda, vo
             long nb)
 {
     return trsm(h, tier, m, n, A, lda, B, ldb, nb);
+}
+
+int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* info)
+{
+    return potrf(h, tier, n, A, lda, nb, info);
 }
 
 int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* A, long lda, void* B,

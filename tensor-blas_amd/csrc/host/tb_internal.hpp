@@ -29,6 +29,7 @@ struct tb_handle_s {
     // slot 1 for the Ozaki GEMM (which routines call with their own slot live)
     void* ws[2] = {nullptr, nullptr};
     size_t ws_bytes[2] = {0, 0};
+    int* dev_info = nullptr;   // POTRF status word (device), allocated on first use
 };
 
 namespace tb {
@@ -41,7 +42,8 @@ enum ProfCat : int {
     kProfTrtri = 12,
     kProfPlanes = 13,          // FP32 -> FP16 plane conversion (work = bytes moved)
     kProfBatch = 14,           // one run of a batch plan (work = its algorithmic flops)
-    kProfCats = 15,
+    kProfPotrfLeaf = 15,       // diagonal-block Cholesky of POTRF (work = s^3/3 flops)
+    kProfCats = 16,
 };
 // Wraps one launch in events when the handle profiles.
 int prof_begin(tb_handle_s* h, hipEvent_t* e0);
@@ -112,6 +114,7 @@ int syr2k(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A,
           const void* B, long ldb, double beta, void* C, long ldc, long nb);
 int trmm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, long lda,
          const void* B, long ldb, void* C, long ldc, long nb);
+int potrf(tb_handle_s* h, int tier, long n, void* A, long lda, long nb, long* info);
 int trsm(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B, long ldb,
          long nb);
 int trsm_refined(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B,

@@ -122,6 +122,13 @@ struct OzakiProb {
 };
 hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);
 
+// Cholesky of one s x s diagonal block (s <= kPotrfLeaf) in place, FP64 arithmetic,
+// lower triangle; A is float (f64 = false) or double.  A failing pivot at local column
+// j sets *info = off + j + 1 if *info is 0; a block with *info != 0 does nothing.
+constexpr int kPotrfLeaf = 128;
+hipError_t potrf_leaf(bool f64, void* A, long long lda, int s, long long off, int* info,
+                      hipStream_t st);
+
 // ---------------------------------------------------------------------- batches
 struct RowexpJob;
 struct TrtriJob;

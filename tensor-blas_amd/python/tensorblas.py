@@ -50,6 +50,7 @@ def lib():
             "tb_trmm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
             "tb_trsm": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
             "tb_trsm_refined": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
+            "tb_potrf": [_P, _I, _L, _P, _L, _L, ctypes.POINTER(ctypes.c_long)],
             "tb_symm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
             "tb_gemm": [_P, _I, _I, _I, _L, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
             "tb_s2h": [_P, _L, _L, _P, _L, _P, _L],
@@ -125,7 +126,8 @@ class Handle:
 
     PROF_NAMES = ["gemm_plain_sq", "gemm_plain_leaf", "gemm_split_sq", "gemm_split_leaf",
                   "gemm_ozaki_sq", "gemm_ozaki_leaf", "gemm_fp32_sq", "gemm_fp32_leaf",
-                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri", "planes", "batch"]
+                  "gemm_fp64_sq", "gemm_fp64_leaf", "ozaki_gemm", "ozaki_slice", "trtri", "planes", "batch",
+                  "potrf_leaf"]
 
     def set_profiling(self, on):
         _check("tb_set_profiling", lib().tb_set_profiling(self._h, int(on)))
@@ -170,6 +172,13 @@ class Handle:
     def trsm_refined(self, tier, m, n, A, lda, B, ldb, nb=256):
         _check("tb_trsm_refined", lib().tb_trsm_refined(self._h, tier_id(tier), m, n, A, lda, B, ldb,
                                                         nb))
+
+    def potrf(self, tier, n, A, lda, nb=256):
+        """Cholesky A = L L^T in place (lower); returns LAPACK's info (0: success, j: the
+        leading minor of order j is not positive definite).  Synchronizes."""
+        info = ctypes.c_long(-1)
+        _check("tb_potrf", lib().tb_potrf(self._h, tier_id(tier), n, A, lda, nb, ctypes.byref(info)))
+        return int(info.value)
 
     def symm(self, tier, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
         _check("tb_symm", lib().tb_symm(self._h, tier_id(tier), m, n, alpha, A, lda, B, ldb, beta, C,

@@ -344,3 +344,35 @@ def test_error_metrics_known_values():
     # lower-only ignores the strictly upper part
     X2 = X.copy(); X2[0, 1] = 100.0
     assert oracle.errors(X2, R, lower=True)["maxrel"] == pytest.approx(0.5 / 4.0)
+
+
+# ------------------------------------------------------------------ Cholesky
+def _spd(g, n):
+    B = g.standard_normal((n, n))
+    return B @ B.T / n + np.eye(n)
+
+
+def test_dpotrf_matches_numpy_cholesky():
+    g = rng(31)
+    for n in (1, 2, 7, 64, 129):
+        A = _spd(g, n)
+        Au = A.copy()
+        Au[np.triu_indices(n, 1)] = np.nan          # the strict upper triangle is never read
+        L, info = oracle.dpotrf_lower(Au)
+        assert info == 0
+        R = np.linalg.cholesky(A)
+        assert np.abs(L - R).max() <= 1e-12 * np.abs(R).max()
+        assert np.abs(L @ L.T - A).max() <= 1e-13 * np.abs(A).max() * n
+        assert not np.triu(L, 1).any()
+
+
+def test_dpotrf_info_reports_first_non_pd_minor():
+    for n, k in ((5, 0), (6, 3), (9, 8)):
+        A = np.eye(n)
+        A[k, k] = -1.0
+        L, info = oracle.dpotrf_lower(A)
+        assert info == k + 1
+        assert np.array_equal(np.diag(L)[:k], np.ones(k))      # steps before k completed
+    v = np.arange(1.0, 5.0)
+    assert oracle.dpotrf_lower(np.outer(v, v))[1] == 2          # rank one: the 2x2 minor is singular
+    assert oracle.dpotrf_lower(np.array([[np.nan]]))[1] == 1
