@@ -69,80 +69,121 @@ __device__ __forceinline__ void rowexp_k_block(const T* __restrict__ X, long lon
     if (lane == 0 && e != kExpZero) atomicMax(out + r, e);
 }
 
+// x = *p if keep, else 0, converted to double, with the load itself unconditional (p
+// must be a valid address either way).  The mask goes through an empty (non-volatile)
+// asm so that the compiler cannot turn the masking back into a branch around the
+// load: a load behind a branch is waited for where the branch rejoins, which
+// serializes a row of loads.
+template <typename T>
+__device__ __forceinline__ double masked_load(const T* p, bool keep)
+{
+    unsigned m = keep ? 0xffffffffu : 0u;
+    asm("" : "+v"(m));
+    if constexpr (sizeof(T) == 4) {
+        return (double)__uint_as_float(*reinterpret_cast<const unsigned*>(p) & m);
+    } else {
+        const unsigned* w = reinterpret_cast<const unsigned*>(p);
+        const unsigned lo = w[0] & m, hi = w[1] & m;
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+}
+
 // Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.
 // Writes W = D inv(L) - I = (I+N)^-1 - I (strictly lower) as Z = W^T (Z[c + r*s] =
-// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One 64-lane block owns the
-// kTrtriCols columns cb.. of inv(L): four lanes per column compute it by forward
-// substitution, each summing every fourth term of a row's dot product (combined by
-// shuffles); the columns live in LDS (y: 256 x kTrtriCols doubles) and row r of L is
-// staged in LDS (lrow: 256) so the inner products read it as a broadcast.  The rows of
-// L do not depend on the results, so each lane loads them kTrtriAhead rows early into
-// a register ring: the row loads' latency overlaps the substitution (s <= 256).  The
-// block is one wave, so LDS results are shared by waiting for the wave's own LDS
-// operations (no barrier, whose fence would also wait for the prefetched rows).  A
-// staged-chunk variant (32 rows of L per column, line-contiguous) measured slower: its
-// larger LDS footprint halves the blocks resident per CU.
+// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One wave owns the kTrtriCols
+// columns cb.. of W and solves them right-looking: lane l holds rows l, l+64, l+128,
+// l+192 of those columns in registers (acc, starting as the identity's columns); at
+// step r the finished row r is broadcast by readlane, v = acc_r / L(r,r), and every
+// later row q subtracts L(q,r) v.  Row q's acc is final once step q is reached and is
+// W(q, .) itself (W = D inv(L) - I needs no final scale), so the wave stores it at the
+// end.  Column r of L is contiguous in memory and is loaded kTrtriAhead steps early
+// into a register ring.  No LDS and no barriers: the chain per step is one readlane
+// broadcast, one division and the row updates (s <= 256).
 constexpr int kTrtriAhead = 8;
-#define TB_WAVE_LDS_SYNC() asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
+
+__device__ __forceinline__ double trtri_readlane(double x, int lane)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 template <typename Tin, typename Tout>
 __device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long long lda, int s,
                                             Tout* __restrict__ Zq, double* __restrict__ dinvq,
-                                            int cb, double* y, double* lrow)
+                                            int cb)
 {
-    const int lane = threadIdx.x;
-    const int cl = lane >> 2, part = lane & 3;
-    const int col = cb + cl;
-    const bool live = col < s;
+    constexpr int C = kTrtriCols;
+    const int lane = threadIdx.x & 63;
     if (cb == 0)
         for (int r = lane; r < s; r += 64) dinvq[r] = 1.0 / (double)Lq[r + (long long)r * lda];
-    for (int r = part; r < cb; r += 4)
-        if (live) Zq[col + (long long)r * s] = Tout(0);
-    // row r of L, columns cb + lane + 64 j (j < 4) with column <= r
-    Tin pre[kTrtriAhead][4];
-    auto fetch = [&](int r, Tin* v) {
+    double acc[4][C];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int c = cb + lane + 64 * j;
-            v[j] = (r < s && c <= r) ? Lq[r + (long long)c * lda] : Tin(0);
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[u][c] = (u * 64 + lane == cb + c) ? 1.0 : 0.0;
+    // column r of L at rows u*64 + lane >= r (others read as zero), and 1 / L(r, r)
+    // (broadcast from the lane holding the diagonal; the division is off the chain)
+    double pre[kTrtriAhead][4], rcpr[kTrtriAhead];
+    // unconditional loads at clamped addresses (masked_load), so a step's loads never
+    // wait for each other
+    auto fetch = [&](int r, double* v) {
+        const int rc = min(r, s - 1);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = u * 64 + lane, qc = min(q, s - 1);
+            v[u] = masked_load(Lq + qc + (long long)rc * lda, r < s && q < s && q >= r);
         }
     };
+    auto diag_rcp = [&](int r, const double* v) {
+        const int ur = (r >> 6) & 3, lr = r & 63;
+        double dg = trtri_readlane(v[0], lr);
 #pragma unroll
-    for (int d = 0; d < kTrtriAhead; ++d) fetch(cb + d, pre[d]);
-    for (int r0 = cb; r0 < s; r0 += kTrtriAhead) {
+        for (int u = 1; u < 4; ++u) dg = ur == u ? trtri_readlane(v[u], lr) : dg;
+        return r < s ? 1.0 / dg : 0.0;
+    };
 #pragma unroll
-        for (int d = 0; d < kTrtriAhead; ++d) {
-            const int r = r0 + d;
-            if (r >= s) break;
+    for (int d = 0; d < kTrtriAhead; ++d) {
+        fetch(cb + d, pre[d]);
+        rcpr[d] = diag_rcp(cb + d, pre[d]);
+    }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int c = cb + lane + 64 * j;
-                if (c <= r) lrow[c - cb] = (double)pre[d][j];
+    for (int u = 0; u < 4; ++u) {
+        // steps r = u*64 + lr owned by slot u (cb is a multiple of kTrtriCols and of 8)
+        const int first = max(cb - u * 64, 0), last = min(64, s - u * 64);
+        for (int lr0 = first; lr0 < last; lr0 += kTrtriAhead) {
+#pragma unroll
+            for (int d = 0; d < kTrtriAhead; ++d) {
+                const int lr = lr0 + d;
+                if (lr < last) {
+                    const int r = u * 64 + lr;
+                    const double rcp = rcpr[d];                                  // 1 / L(r, r)
+                    double v[C];
+#pragma unroll
+                    for (int c = 0; c < C; ++c) v[c] = trtri_readlane(acc[u][c], lr) * rcp;
+#pragma unroll
+                    for (int w = u; w < 4; ++w) {
+                        const double lq = pre[d][w];             // L(w*64 + lane, r), 0 unless below r
+                        if (w > u || lane > lr) {
+#pragma unroll
+                            for (int c = 0; c < C; ++c) acc[w][c] -= lq * v[c];
+                        }
+                    }
+                    fetch(r + kTrtriAhead, pre[d]);
+                    rcpr[d] = diag_rcp(r + kTrtriAhead, pre[d]);
+                }
             }
-            fetch(r + kTrtriAhead, pre[d]);
-            TB_WAVE_LDS_SYNC();
-            // every fourth term of the row's dot product, in four independent partial
-            // sums so that several LDS reads are in flight at once
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            int p = cb + part;
-            for (; p + 12 < r; p += 16) {
-                s0 += lrow[p - cb] * y[(p - cb) * kTrtriCols + cl];
-                s1 += lrow[p + 4 - cb] * y[(p + 4 - cb) * kTrtriCols + cl];
-                s2 += lrow[p + 8 - cb] * y[(p + 8 - cb) * kTrtriCols + cl];
-                s3 += lrow[p + 12 - cb] * y[(p + 12 - cb) * kTrtriCols + cl];
-            }
-            for (; p < r; p += 4) s0 += lrow[p - cb] * y[(p - cb) * kTrtriCols + cl];
-            double sum = (s0 + s1) + (s2 + s3);
-            sum += __shfl_xor(sum, 1);
-            sum += __shfl_xor(sum, 2);
-            const double dg = lrow[r - cb];
-            const double v = (r < col || !live) ? 0.0 : (((r == col) ? 1.0 : 0.0) - sum) / dg;
-            if (part == 0) {
-                y[(r - cb) * kTrtriCols + cl] = v;
-                // W(r, col) = L(r,r) * inv(L)(r, col) for r > col
-                if (live) Zq[col + (long long)r * s] = (r > col) ? (Tout)(dg * v) : Tout(0);
-            }
-            TB_WAVE_LDS_SYNC();
+        }
+    }
+    // row q of W: acc (q > column), zero on and above the diagonal
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int q = u * 64 + lane;
+        if (q < s) {
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                if (cb + c < s) Zq[(cb + c) + (long long)q * s] = (q > cb + c) ? (Tout)acc[u][c] : Tout(0);
         }
     }
 }

@@ -40,10 +40,8 @@ __global__ void __launch_bounds__(64)
 trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride, int s,
                      Tout* __restrict__ Z, double* __restrict__ dinv)
 {
-    __shared__ double y[256 * kTrtriCols];
-    __shared__ double lrow[256];
     trtri_block(L + (long long)blockIdx.y * bstride, lda, s, Z + (long long)blockIdx.y * s * s,
-                dinv + (long long)blockIdx.y * s, blockIdx.x * kTrtriCols, y, lrow);
+                dinv + (long long)blockIdx.y * s, blockIdx.x * kTrtriCols);
 }
 
 // FP32 -> FP16, round to nearest even

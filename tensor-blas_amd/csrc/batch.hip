@@ -267,13 +267,11 @@ batch_rowexp_kernel(const RowexpJob* __restrict__ jobs, int njobs)
 __global__ void __launch_bounds__(64)
 batch_trtri_kernel(const TrtriJob* __restrict__ jobs, int njobs)
 {
-    __shared__ double y[256 * kTrtriCols];
-    __shared__ double lrow[256];
     const TrtriJob& J = find_job(jobs, njobs, blockIdx.x);
     const int local = blockIdx.x - J.blockBegin;
     const int q = local / J.colBlocks, cb = (local % J.colBlocks) * kTrtriCols;
     trtri_block(J.L + (long long)q * J.bstride, J.lda, J.s, J.Z + (long long)q * J.s * J.s,
-                J.dinv + (long long)q * J.s, cb, y, lrow);
+                J.dinv + (long long)q * J.s, cb);
 }
 
 __global__ void __launch_bounds__(256)

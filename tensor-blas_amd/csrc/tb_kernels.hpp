@@ -69,8 +69,8 @@ hipError_t fill_i32(int* p, long long n, int v, hipStream_t s);
 // D inv(L_q) - I (strictly lower; Z_q dense s x s, ld s, at Z + q*s*s) and
 // dinv[q*s + r] = 1 / L_q(r, r).  Then X L_q^T = B  <=>  X = (B + B W^T) D^-1.
 // FP64 arithmetic; Z is FP32 unless out_f64.  s <= 256.
-// columns of inv(L) per 64-lane block of the diagonal-block inversion
-constexpr int kTrtriCols = 16;
+// columns of inv(L) per 64-lane block (one wave) of the diagonal-block inversion
+constexpr int kTrtriCols = 8;
 hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda,
                          long long bstride, int s, int nblocks, void* Z, double* dinv,
                          hipStream_t st);
