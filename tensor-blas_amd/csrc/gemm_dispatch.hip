@@ -32,13 +32,15 @@ hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s, int mc)
     // stages (<= 4) as fit in the 160 KB of LDS.  Square and leaf tiles: 64 x 64
     // accumulators per wave.  Wide tiles: 128 x 64 per wave, which reads 25 % fewer LDS
     // bytes per MFMA; at 2 planes its two accumulator sets fill the 256 AGPRs.
-    if (shape == kShapeLeaf)
+    if (shape == kShapeLeaf) {
+        if constexpr (NPL == 2) return run16<NPL, SA, SB, 64, 256, 1, 8, 3>(L, s, mc);
         return run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s, mc);
+    }
     if constexpr (NPL <= 2 && SA == kSrcPlanes && SB == kSrcPlanes) {
-        if (shape == kShapeWide) return run16<NPL, SA, SB, 256, 128, 2, 2, NPL == 2 ? 3 : 4>(L, s, 0);
+        if (shape == kShapeWide) return run16<NPL, SA, SB, 256, 128, 4, 2, NPL == 2 ? 3 : 4>(L, s, 0);
     }
     if (shape == kShapeWide) return hipErrorInvalidValue;
-    return run16<NPL, SA, SB, 128, 128, 2, 2, NPL == 3 ? 3 : 4>(L, s, mc);
+    return run16<NPL, SA, SB, 128, 128, 2, 4, NPL == 3 ? 3 : 4>(L, s, mc);
 }
 
 template <int NPL>
