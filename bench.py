@@ -440,23 +440,29 @@ def run_ozaki(args, torch, tensorblas):
         return {"maxrel": worst, "normwise": (num / den) ** 0.5}
 
     res = {}
-    for s in [int(x) for x in args.slices.split(",")]:
+    # "8" = 8 slices, triangular scheme; "8+" = 8 slices plus the slice-pair group g = 8
+    for tok in args.slices.split(","):
+        s, extra = int(tok.rstrip("+")), int(tok.endswith("+"))
         h.set_ozaki_slices(s)
+        h.set_ozaki_extra(extra)
         ms = timed(h, step("ozaki"), args.steps, args.warmup)
-        res[f"ozaki_s{s}"] = {"ms": round(ms, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
-                              "errors_vs_ref": col_err()}
+        res[f"ozaki_s{tok}"] = {"ms": round(ms, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                                "errors_vs_ref": col_err()}
     ms64 = timed(h, step("fp64"), 2, 1)
     res["fp64_mfma"] = {"ms": round(ms64, 2), "tflops": round(flops / (ms64 * 1e-3) / 1e12, 2),
                         "errors_vs_ref": col_err()}
     ok = [(v["ms"], key) for key, v in res.items()
           if key != "fp64_mfma" and v["errors_vs_ref"]["maxrel"] <= 1e-14]
     best = min(ok) if ok else min((v["ms"], key) for key, v in res.items() if key != "fp64_mfma")
-    sbest = int(best[1].split("_s")[1])
+    tbest = best[1].split("_s")[1]
+    sbest, xbest = int(tbest.rstrip("+")), int(tbest.endswith("+"))
     h.set_ozaki_slices(sbest)
+    h.set_ozaki_extra(xbest)
     prof = profiled(h, step("ozaki"))
     oz = {k_: v for k_, v in prof.items() if k_ == "ozaki_gemm"}
     roof, rk = roofline(oz, FP64_PEAK_TF, "TFLOP/s", "mfma", f"{op}{n}x{k}",
-                        f"FP64-equivalent flops; {sbest} slices run {sbest * (sbest + 1) // 2} "
+                        f"FP64-equivalent flops; {sbest} slices{' + group g=S' if xbest else ''} run "
+                        f"{sbest * (sbest + 1) // 2 + xbest * (sbest - 1)} "
                         f"INT8 MFMA slice products per flop (INT8 peak {INT8_PEAK_TOPS} TOPS)")
     ms = best[0]
     line = line_base(f"eff. FP64 TFLOP/s of Ozaki {op.upper()} at <=1e-14 max relative error",
@@ -464,7 +470,7 @@ def run_ozaki(args, torch, tensorblas):
                      "f64 (int8 slices, fp64 accumulate)",
                      f"synthetic: U[-0.5,0.5]*exp({phi}*N(0,1)) FP64 (seeded)",
                      {"workload": f"{op.upper()}, Ozaki INT8 slicing", **shape, "phi": phi,
-                      "slices": sbest})
+                      "slices": sbest, "extra_group": xbest})
     line.update({"results": res, "vs_fp64_mfma": round(ms64 / ms, 3), "meets_1e-14": bool(ok),
                  "roofline": roof, "roofline_kernel": rk,
                  "profile_ms": {k_: round(v[1], 3) for k_, v in prof.items()},

@@ -78,6 +78,12 @@ TB_API int tb_set_stream(tb_handle_t handle, void* stream);
 TB_API int tb_synchronize(tb_handle_t handle);
 /* Ozaki tier: number of INT8 slices (3..10, default 7) */
 TB_API int tb_set_ozaki_slices(tb_handle_t handle, int nslices);
+/* Ozaki tier: 1 also sums the slice pairs (s, t) with s + t = nslices (0-based), the
+ * first group the triangular scheme drops: nslices - 1 more INT8 products on the same
+ * slices, for about the accuracy of one more slice without its memory traffic.
+ * 0 (default) is the published scheme.  With extra = 1, nslices must be 6..9.  No
+ * counterpart in the reference. */
+TB_API int tb_set_ozaki_extra(tb_handle_t handle, int extra);
 /* Split tier: FP16 planes per FP32 operand (default 2).  2: x*2^s = h + 2^-11 m
  * (23 significant bits for each row's largest entries), three MFMA products
  * (hh | hm+mh): the published FP16 error-correction scheme.  3: h + 2^-11 m + 2^-22 l,

@@ -50,7 +50,7 @@ def lib():
         L.orc_tc_trsm_plain_emul.argtypes = [_i64, _i64, _p, _i64, _p, _i64, _i64, _p]
         L.orc_ozaki_slice.argtypes = [ctypes.c_int, _i64, _i64, _p, _i64, ctypes.c_int, _p, _p]
         L.orc_ozaki_gemm.argtypes = [_i64, _i64, _i64, ctypes.c_int, _p, _p, _p, _p, _dbl,
-                                     _dbl, _p, _i64, _i64]
+                                     _dbl, _p, _i64, _i64, ctypes.c_int]
         L.orc_errors.argtypes = [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, ctypes.c_int, _p]
         L.orc_dpotrf_lower.argtypes = [_i64, _p, _i64]
         L.orc_dpotrf_lower.restype = ctypes.c_int
@@ -191,16 +191,16 @@ def ozaki_slice(X, nslices, trans=False):
     return sl, ex
 
 
-def ozaki_gemm(sa, ea, sb, eb, alpha=1.0, beta=0.0, C=None, kchunk=0):
+def ozaki_gemm(sa, ea, sb, eb, alpha=1.0, beta=0.0, C=None, kchunk=0, ngroups=0):
     """C = alpha * (A_sliced * B_sliced^T) + beta*C from row slices of A (m x k) and
-    row slices of B^T (n x k)."""
+    row slices of B^T (n x k); slice-pair groups g < ngroups (default: nslices)."""
     ns, m, k = sa.shape
     n = sb.shape[1]
     sa = np.ascontiguousarray(sa); sb = np.ascontiguousarray(sb)
     ea = np.ascontiguousarray(ea, dtype=np.int32); eb = np.ascontiguousarray(eb, dtype=np.int32)
     C = np.zeros((m, n), order="F") if C is None else _f64(C).copy(order="F")
     lib().orc_ozaki_gemm(m, n, k, ns, _ptr(sa), _ptr(ea), _ptr(sb), _ptr(eb), alpha, beta,
-                         _ptr(C), max(1, m), kchunk)
+                         _ptr(C), max(1, m), kchunk, int(ngroups))
     return C
 
 

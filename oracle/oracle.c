@@ -365,26 +365,30 @@ void orc_ozaki_slice(int trans, int64_t m, int64_t k, const double *X, int64_t l
 /* Ozaki product from slices.  With g = s+t, the slice pairs of one g share the
  * weight 2^(-7(g+2)), so their integer products are summed together:
  *     D_g(i,j) = sum_{s+t=g} sum_{p in chunk} a_s(i,p) * b_t(j,p)     (exact integer)
- * for g = 0..nslices-1 (pairs with s+t >= nslices are dropped: the triangular
- * truncation of the Ozaki scheme).  For each k chunk in order, and g in increasing
+ * for g = 0..ngroups-1 with s, t < nslices (ngroups = nslices: pairs with
+ * s+t >= nslices are dropped, the triangular truncation of the Ozaki scheme; the
+ * library's option ngroups = nslices + 1 also keeps the group g = nslices, which has
+ * no counterpart in the published scheme).  For each k chunk in order, and g in increasing
  * order, acc += D_g * 2^(-7(g+2)) in binary64.  Finally v = acc * 2^(ea_i+eb_j)
  * and C = fma(alpha, v, beta*C) (alpha*v when beta == 0).  This is the exact
  * operation order of the device kernel, so results agree bit for bit.  kchunk
  * keeps each D_g inside int32 on the device: nslices*kchunk*127^2 < 2^31. */
 void orc_ozaki_gemm(int64_t m, int64_t n, int64_t k, int nslices, const int8_t *sa,
                     const int32_t *ea, const int8_t *sb, const int32_t *eb, double alpha,
-                    double beta, double *C, int64_t ldc, int64_t kchunk)
+                    double beta, double *C, int64_t ldc, int64_t kchunk, int ngroups)
 {
     if (kchunk <= 0) kchunk = k;
+    if (ngroups <= 0) ngroups = nslices;
     for (int64_t j = 0; j < n; j++)
         for (int64_t i = 0; i < m; i++) {
             double acc = 0.0;
             for (int64_t k0 = 0; k0 < k; k0 += kchunk) {
                 int64_t k1 = k0 + kchunk < k ? k0 + kchunk : k;
-                for (int g = 0; g < nslices; g++) {
+                for (int g = 0; g < ngroups; g++) {
                     int64_t dot = 0;
                     for (int s = 0; s <= g; s++) {
                         int t = g - s;
+                        if (s >= nslices || t >= nslices) continue;
                         const int8_t *a = sa + ((int64_t)s * m + i) * k;
                         const int8_t *b = sb + ((int64_t)t * n + j) * k;
                         for (int64_t p = k0; p < k1; p++) dot += (int64_t)a[p] * b[p];

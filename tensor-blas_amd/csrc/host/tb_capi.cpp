@@ -89,6 +89,13 @@ int tb_set_ozaki_slices(tb_handle_t h, int nslices)
     return TB_SUCCESS;
 }
 
+int tb_set_ozaki_extra(tb_handle_t h, int extra)
+{
+    if (!h || extra < 0 || extra > 1) return TB_ERR_INVALID;
+    h->ozaki_extra = extra;
+    return TB_SUCCESS;
+}
+
 int tb_set_max_cus(tb_handle_t h, int max_cus)
 {
     if (!h || max_cus < 0) return TB_ERR_INVALID;

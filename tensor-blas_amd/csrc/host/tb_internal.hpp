@@ -23,6 +23,7 @@ struct tb_handle_s {
     std::vector<tb_prof_rec> prof_recs;
     int device = 0;
     int ozaki_slices = 7;
+    int ozaki_extra = 0;    // 1: also the slice-pair group g = nslices (6 <= nslices <= 9)
     int split_planes = 2;   // FP16 planes per operand of the split tier (2 or 3)
     int max_cus = 0;        // > 0: GEMM launches occupy at most this many CUs
     // device workspaces, grown on demand, freed by tb_destroy: slot 0 for routines,

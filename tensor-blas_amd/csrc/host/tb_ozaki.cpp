@@ -25,6 +25,7 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
 {
     if (m <= 0 || n <= 0) return TB_SUCCESS;
     const int S = h->ozaki_slices;
+    if (h->ozaki_extra && (S < 6 || S > 9)) return TB_ERR_INVALID;   // kernels instantiated
     const long kpad = std::max(64l, (k + 63) / 64 * 64);
     const bool pairs = A2 != nullptr;
     Arena a;
@@ -79,6 +80,7 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     p.n = (int)n;
     p.kpad = (int)kpad;
     p.nslices = S;
+    p.ngroups = S + h->ozaki_extra;
     p.kchunk = (int)ozaki_kchunk(S);
     p.lowerC = lowerC ? 1 : 0;
     const double flops = (lowerC ? (double)n * (n + 1) * k : 2.0 * m * n * k) * (pairs ? 2 : 1);

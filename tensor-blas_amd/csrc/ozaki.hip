@@ -10,6 +10,8 @@
 // The S(S+1)/2 slice-pair products run on v_mfma_i32_32x32x32_i8; all pairs of one g
 // share an int32 accumulator (exact while S*kchunk*127^2 < 2^31, K is processed in
 // chunks), and each chunk's D_g are added to an FP64 accumulator in increasing g.
+// Optionally (G = S + 1 groups) the next anti-diagonal g = S is kept too: S - 1 more
+// products on the same slices, which removes the largest dropped term.
 // The operation order is exactly oracle/oracle.c orc_ozaki_gemm's, so the result
 // is reproducible bit for bit.
 #include <hip/hip_runtime.h>
@@ -160,7 +162,7 @@ __device__ __forceinline__ void lower_tile_banded(int g, int T, int& ti, int& tj
 // so both the fragment reads (16 lanes: 16 rows of one unit) and the 1 KB direct-to-LDS
 // writes cover the 64 banks once.  The ring holds NST stages; each stage's loads are
 // issued NST-1 stages ahead of its MFMAs.
-template <int S>
+template <int S, int G>
 __global__ void __launch_bounds__(256)
 ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
 {
@@ -241,12 +243,12 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
             for (int q = 0; q < DMA_W; ++q) oz_dma16(gsrc[q] + kb * (isB[q] ? kstepB : kstepA), base + ldo[q]);
         };
 
-        intx16_t acc[S];
+        intx16_t acc[G];
         double facc[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) facc[r] = 0.0;
 #pragma unroll
-        for (int q = 0; q < S; ++q) acc[q] = intx16_t{};
+        for (int q = 0; q < G; ++q) acc[q] = intx16_t{};
 
         const int cblocks = P.kchunk / OBK;          // k blocks per exactness chunk
         // fragment of lane: row (panel base + lane%32), logical unit lane/32
@@ -275,18 +277,19 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
             // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
             const bool ahead = b + NST - 1 < nblk;
             if (ahead) issue(b + NST - 1);
-            // D_g += A_s B_{g-s}: consecutive MFMAs update different D_g
+            // D_g += A_s B_{g-s} (s, g-s < S; g < G): consecutive MFMAs update different D_g
 #pragma unroll
             for (int s = 0; s < S; ++s) {
 #pragma unroll
-                for (int gg = s; gg < S; ++gg)
-                    acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
+                for (int gg = s; gg < G; ++gg)
+                    if (gg - s < S)
+                        acc[gg] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[gg - s], af[s], acc[gg], 0, 0, 0);
             }
             const int kb = b % kblocks;
             if ((kb + 1) % cblocks == 0 || kb + 1 == kblocks) {
                 // exactness chunk done: D_g into FP64, increasing g
 #pragma unroll
-                for (int gg = 0; gg < S; ++gg) {
+                for (int gg = 0; gg < G; ++gg) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r)
                         facc[r] = facc[r] + __builtin_ldexp((double)acc[gg][r], -7 * (gg + 2));
@@ -335,21 +338,23 @@ hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus)
 {
     if (p.m <= 0 || p.n <= 0) return hipSuccess;
     if (p.nslices < kMinSlices || p.nslices > kMaxSlices || p.kpad % OBK || p.kchunk < OBK ||
-        p.kchunk % OBK)
+        p.kchunk % OBK || (p.ngroups != p.nslices && p.ngroups != p.nslices + 1))
         return hipErrorInvalidValue;
     const int tm = (p.m + OBM - 1) / OBM, tn = (p.n + OBN - 1) / OBN;
     if (p.lowerC && (p.m != p.n)) return hipErrorInvalidValue;
     const int total = p.lowerC ? tm * (tm + 1) / 2 : tm * tn;
-    switch (p.nslices) {
-#define TB_OZ_CASE(S_)                                                                            \
-    case S_:                                                                                      \
-        hipLaunchKernelGGL(ozaki_gemm_kernel<S_>,                                                 \
-                           dim3(capped_grid(reinterpret_cast<const void*>(ozaki_gemm_kernel<S_>), \
+    const int key = p.nslices * 2 + (p.ngroups - p.nslices);
+    switch (key) {
+#define TB_OZ_CASE(S_, E_)                                                                        \
+    case S_ * 2 + E_:                                                                             \
+        hipLaunchKernelGGL((ozaki_gemm_kernel<S_, S_ + E_>),                                      \
+                           dim3(capped_grid(reinterpret_cast<const void*>(ozaki_gemm_kernel<S_, S_ + E_>), \
                                             256, total, max_cus)),                                \
                            dim3(256), 0, s, p, tm, total);                                        \
         break;
-        TB_OZ_CASE(3) TB_OZ_CASE(4) TB_OZ_CASE(5) TB_OZ_CASE(6) TB_OZ_CASE(7) TB_OZ_CASE(8)
-        TB_OZ_CASE(9) TB_OZ_CASE(10)
+        TB_OZ_CASE(3, 0) TB_OZ_CASE(4, 0) TB_OZ_CASE(5, 0) TB_OZ_CASE(6, 0) TB_OZ_CASE(7, 0)
+        TB_OZ_CASE(8, 0) TB_OZ_CASE(9, 0) TB_OZ_CASE(10, 0)
+        TB_OZ_CASE(6, 1) TB_OZ_CASE(7, 1) TB_OZ_CASE(8, 1) TB_OZ_CASE(9, 1)
 #undef TB_OZ_CASE
     default: return hipErrorInvalidValue;
     }

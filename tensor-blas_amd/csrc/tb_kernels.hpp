@@ -118,6 +118,7 @@ struct OzakiProb {
     long long ldc;
     double alpha, beta;
     int m, n, kpad, nslices, kchunk;
+    int ngroups;        // slice-pair groups g = 0 .. ngroups-1 kept: nslices or nslices + 1
     int lowerC;
 };
 hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);

@@ -42,7 +42,7 @@ def lib():
         L = ctypes.CDLL(path)
         sig = {
             "tb_create": [_P], "tb_destroy": [_P], "tb_set_stream": [_P, _P],
-            "tb_synchronize": [_P], "tb_set_ozaki_slices": [_P, _I],
+            "tb_synchronize": [_P], "tb_set_ozaki_slices": [_P, _I], "tb_set_ozaki_extra": [_P, _I],
             "tb_set_split_planes": [_P, _I],
             "tb_set_max_cus": [_P, _I],
             "tb_syrk": [_P, _I, _L, _L, _D, _P, _L, _D, _P, _L, _L],
@@ -115,6 +115,11 @@ class Handle:
 
     def set_ozaki_slices(self, s):
         _check("tb_set_ozaki_slices", lib().tb_set_ozaki_slices(self._h, s))
+
+    def set_ozaki_extra(self, extra):
+        """1: also sum the slice-pair group g = nslices (nslices 6..9); 0: the published
+        triangular scheme."""
+        _check("tb_set_ozaki_extra", lib().tb_set_ozaki_extra(self._h, int(extra)))
 
     def set_split_planes(self, n):
         _check("tb_set_split_planes", lib().tb_set_split_planes(self._h, n))

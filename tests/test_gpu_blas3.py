@@ -292,13 +292,14 @@ def _oz_slices(X, S):
     return oracle.ozaki_slice(X, S)
 
 
-@pytest.mark.parametrize("S", [3, 6, 7, 9])
+@pytest.mark.parametrize("S,extra", [(3, 0), (6, 0), (7, 0), (9, 0), (6, 1), (8, 1), (9, 1)])
 @pytest.mark.parametrize("m,n,k,phi", [(17, 9, 33, 0.5), (70, 130, 200, 2.0), (64, 64, 64, 0.5)])
-def test_ozaki_gemm_bit_exact(h, S, m, n, k, phi):
+def test_ozaki_gemm_bit_exact(h, S, extra, m, n, k, phi):
     g = rng(S * 101 + m + n + k)
     A, B = spread(g, (m, k), phi), spread(g, (k, n), phi)
     C0 = spread(g, (m, n), phi)
     h.set_ozaki_slices(S)
+    h.set_ozaki_extra(extra)
     dA, lda = to_dev(A, m + 1)
     dB, ldb = to_dev(B, k)
     dC, ldc = to_dev(C0, m)
@@ -308,9 +309,22 @@ def test_ozaki_gemm_bit_exact(h, S, m, n, k, phi):
     C = from_dev(dC, m, n, ldc)
     sa, ea = _oz_slices(A, S)
     sb, eb = _oz_slices(B.T.copy(), S)
-    R = oracle.ozaki_gemm(sa, ea, sb, eb, 1.25, -0.5, C0, kchunk=_ozaki_kchunk(S))
-    assert np.array_equal(C, R), np.max(np.abs(C - R))
+    R = oracle.ozaki_gemm(sa, ea, sb, eb, 1.25, -0.5, C0, kchunk=_ozaki_kchunk(S), ngroups=S + extra)
+    h.set_ozaki_extra(0)
     h.set_ozaki_slices(7)
+    assert np.array_equal(C, R), np.max(np.abs(C - R))
+
+
+def test_ozaki_extra_group_arguments(h, tbm):
+    h.set_ozaki_slices(4)
+    h.set_ozaki_extra(1)                      # accepted; the call then refuses nslices 4
+    z = torch.zeros(16, device="cuda", dtype=torch.float64)
+    with pytest.raises(tbm.TensorBLASError):
+        h.gemm("ozaki", 0, 0, 4, 4, 4, 1.0, z.data_ptr(), 4, z.data_ptr(), 4, 0.0, z.data_ptr(), 4)
+    h.set_ozaki_extra(0)
+    h.set_ozaki_slices(7)
+    with pytest.raises(tbm.TensorBLASError):
+        h.set_ozaki_extra(2)
 
 
 def test_ozaki_routines_bit_exact(h):

@@ -376,3 +376,29 @@ def test_dpotrf_info_reports_first_non_pd_minor():
     v = np.arange(1.0, 5.0)
     assert oracle.dpotrf_lower(np.outer(v, v))[1] == 2          # rank one: the 2x2 minor is singular
     assert oracle.dpotrf_lower(np.array([[np.nan]]))[1] == 1
+
+
+def test_ozaki_gemm_all_groups_is_exact_product_of_slices():
+    """With every slice-pair group kept (ngroups = 2S - 1) the product is that of the
+    sliced representations themselves, so it matches exact rational arithmetic on them
+    up to the binary64 roundings of the group sums; keeping one group more than the
+    triangular scheme (ngroups = S + 1) lies between the two."""
+    g = rng(17)
+    m, n, k, S = 4, 5, 40, 3
+    A = g.standard_normal((m, k)) * np.exp2(g.integers(-20, 20, (m, 1)).astype(float))
+    B = g.standard_normal((n, k)) * np.exp2(g.integers(-20, 20, (n, 1)).astype(float))
+    sa, ea = oracle.ozaki_slice(A, S)
+    sbs, eb = oracle.ozaki_slice(B, S)
+    w = [Fraction(1, 2 ** (7 * (s + 1))) for s in range(S)]
+    def rec(sl, ex, i, p):
+        return sum(w[s] * int(sl[s, i, p]) for s in range(S)) * Fraction(2) ** int(ex[i])
+    full = oracle.ozaki_gemm(sa, ea, sbs, eb, ngroups=2 * S - 1)
+    tri = oracle.ozaki_gemm(sa, ea, sbs, eb)
+    plus = oracle.ozaki_gemm(sa, ea, sbs, eb, ngroups=S + 1)
+    for i in range(m):
+        for j in range(n):
+            exact = sum(rec(sa, ea, i, p) * rec(sbs, eb, j, p) for p in range(k))
+            scale = sum(abs(rec(sa, ea, i, p) * rec(sbs, eb, j, p)) for p in range(k))
+            assert abs(Fraction(full[i, j]) - exact) <= scale * Fraction(1, 2 ** 50)
+            assert abs(plus[i, j] - float(exact)) <= abs(tri[i, j] - float(exact)) + float(scale) * 2 ** -50
+
