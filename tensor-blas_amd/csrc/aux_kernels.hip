@@ -241,6 +241,53 @@ hipError_t s2h(long long m, long long n, const float* as, long long ldas, void* 
     return hipGetLastError();
 }
 
+namespace {
+
+// one 32 x 32 tile of dst per block (32 x 8 threads): the src block it maps from is
+// read column by column into LDS, then dst is written column by column
+template <typename T>
+__global__ void __launch_bounds__(256)
+xform_kernel(const T* __restrict__ src, long long lds, T* __restrict__ dst, long long ldd, int rows,
+             int cols, int flags, T alpha)
+{
+    __shared__ T t[32][33];
+    const bool tr = flags & kXformTrans, rr = flags & kXformRevRows, rc = flags & kXformRevCols;
+    const int bi = blockIdx.x * 32, bj = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // dst rows bi.. map to i' in [a0, a0+31], dst cols bj.. to j' in [b0, b0+31]
+    const int a0 = rr ? rows - bi - 32 : bi, b0 = rc ? cols - bj - 32 : bj;
+    const int sr0 = tr ? b0 : a0, sc0 = tr ? a0 : b0;             // src block origin
+    const int srows = tr ? cols : rows, scols = tr ? rows : cols;
+    for (int y = ty; y < 32; y += 8) {
+        const int si = sr0 + tx, sj = sc0 + y;
+        t[y][tx] = (si >= 0 && si < srows && sj >= 0 && sj < scols) ? src[si + (long long)sj * lds] : T(0);
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        const int i = bi + tx, j = bj + y;
+        if (i >= rows || j >= cols) continue;
+        const int ip = (rr ? rows - 1 - i : i) - a0, jp = (rc ? cols - 1 - j : j) - b0;   // in [0, 32)
+        const T v = tr ? t[ip][jp] : t[jp][ip];
+        dst[i + (long long)j * ldd] = ((flags & kXformUnitDiag) && i == j) ? T(1) : alpha * v;
+    }
+}
+
+}  // namespace
+
+hipError_t xform(bool f64, const void* src, long long lds, void* dst, long long ldd, int rows, int cols,
+                 int flags, double alpha, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    const dim3 g((rows + 31) / 32, (cols + 31) / 32);
+    if (f64)
+        hipLaunchKernelGGL(xform_kernel<double>, g, dim3(256), 0, s, (const double*)src, lds,
+                           (double*)dst, ldd, rows, cols, flags, alpha);
+    else
+        hipLaunchKernelGGL(xform_kernel<float>, g, dim3(256), 0, s, (const float*)src, lds,
+                           (float*)dst, ldd, rows, cols, flags, (float)alpha);
+    return hipGetLastError();
+}
+
 hipError_t s2h_transpose(long long m, long long n, const float* as, void* ah, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return hipSuccess;

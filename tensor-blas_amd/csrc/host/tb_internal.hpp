@@ -27,9 +27,10 @@ struct tb_handle_s {
     int split_planes = 2;   // FP16 planes per operand of the split tier (2 or 3)
     int max_cus = 0;        // > 0: GEMM launches occupy at most this many CUs
     // device workspaces, grown on demand, freed by tb_destroy: slot 0 for routines,
-    // slot 1 for the Ozaki GEMM (which routines call with their own slot live)
-    void* ws[2] = {nullptr, nullptr};
-    size_t ws_bytes[2] = {0, 0};
+    // slot 1 for the Ozaki GEMM (which routines call with their own slot live), slot 2
+    // for the BLAS-option layer's transformed operands (live across a routine call)
+    void* ws[3] = {nullptr, nullptr, nullptr};
+    size_t ws_bytes[3] = {0, 0, 0};
     int* dev_info = nullptr;   // POTRF status word (device), allocated on first use
 };
 
@@ -116,6 +117,18 @@ int syr2k(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A,
 int trmm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, long lda,
          const void* B, long ldb, void* C, long ldc, long nb);
 int potrf(tb_handle_s* h, int tier, long n, void* A, long lda, long nb, long* info);
+// BLAS option sets (side 'L'/'R', uplo 'L'/'U', trans 'N'/'T'/'C', diag 'N'/'U'),
+// mapped onto the native forms above by transposed and reversed operand copies
+int trsm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char diag, long m, long n,
+            double alpha, const void* A, long lda, void* B, long ldb, long nb);
+int trmm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char diag, long m, long n,
+            double alpha, const void* A, long lda, const void* B, long ldb, void* C, long ldc, long nb);
+int symm_ex(tb_handle_s* h, int tier, char side, char uplo, long m, long n, double alpha, const void* A,
+            long lda, const void* B, long ldb, double beta, void* C, long ldc);
+int syrk_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+            long lda, double beta, void* C, long ldc, long nb);
+int syr2k_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+             long lda, const void* B, long ldb, double beta, void* C, long ldc, long nb);
 int trsm(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B, long ldb,
          long nb);
 int trsm_refined(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B,

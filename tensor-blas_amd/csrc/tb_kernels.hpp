@@ -83,6 +83,20 @@ hipError_t s2h(long long m, long long n, const float* as, long long ldas, void* 
 hipError_t s2h_transpose(long long m, long long n, const float* as, void* ah, hipStream_t s);
 // A(i,j) := A(j,i) for i < j (util/util.cu:148-167)
 hipError_t mirror_lower(bool f64, long long n, void* A, long long lda, hipStream_t s);
+
+// dst (rows x cols, ld ldd) := alpha * src under an index map: with
+// i' = kXformRevRows ? rows-1-i : i and j' = kXformRevCols ? cols-1-j : j, dst(i, j) =
+// src(j', i') if kXformTrans else src(i', j'); kXformUnitDiag makes dst's diagonal 1.
+// src and dst must not overlap.  Float or double (f64).  Tiled through LDS so that
+// both sides are read and written in columns.
+enum XformFlags : int {
+    kXformTrans = 1,
+    kXformRevRows = 2,
+    kXformRevCols = 4,
+    kXformUnitDiag = 8,
+};
+hipError_t xform(bool f64, const void* src, long long lds, void* dst, long long ldd, int rows, int cols,
+                 int flags, double alpha, hipStream_t s);
 // a(i,j) := v (util/util.cu:240-257)
 hipError_t fill_matrix(bool f64, long long m, long long n, void* a, long long lda, double v,
                        hipStream_t s);
