@@ -150,8 +150,10 @@ TB_API int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* 
  * double* A for ozaki/fp64; nb is passed to tb_trsm and tb_syrk. */
 TB_API int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* info);
 /* The BLAS option sets around the fixed forms above (no counterpart in the reference,
- * whose routines take one option set each).  Options are BLAS characters: side 'L'/'R',
- * uplo 'L'/'U', transa 'N'/'T' ('C' = 'T' for real data), diag 'N'/'U'.  Semantics:
+ * whose routines take one option set each).  Options are BLAS characters: order 'C'
+ * (column-major) or 'R' (row-major, as CBLAS: every matrix argument and its leading
+ * dimension in row-major storage), side 'L'/'R', uplo 'L'/'U', transa 'N'/'T' ('C' = 'T'
+ * for real data), diag 'N'/'U'.  Semantics:
  *   tb_trsm_ex:  op(A) X = alpha B (side L) or X op(A) = alpha B (side R); B := X
  *   tb_trmm_ex:  C = alpha op(A) B (side L) or alpha B op(A) (side R), out of place
  *                (as tc_trmm); A triangular with uplo/diag
@@ -163,18 +165,21 @@ TB_API int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb,
  * Each maps onto the native form by transposed / reversed operand copies in the
  * handle's workspace (O(n^2) or O(mn) extra traffic), so results carry the tier's
  * accuracy. */
-TB_API int tb_trsm_ex(tb_handle_t h, int tier, char side, char uplo, char transa, char diag, long m, long n,
-                      double alpha, const void* A, long lda, void* B, long ldb, long nb);
-TB_API int tb_trmm_ex(tb_handle_t h, int tier, char side, char uplo, char transa, char diag, long m, long n,
-                      double alpha, const void* A, long lda, const void* B, long ldb, void* C, long ldc,
+TB_API int tb_trsm_ex(tb_handle_t h, int tier, char order, char side, char uplo, char transa,
+                      char diag, long m, long n, double alpha, const void* A, long lda, void* B,
+                      long ldb, long nb);
+TB_API int tb_trmm_ex(tb_handle_t h, int tier, char order, char side, char uplo, char transa,
+                      char diag, long m, long n, double alpha, const void* A, long lda,
+                      const void* B, long ldb, void* C, long ldc, long nb);
+TB_API int tb_symm_ex(tb_handle_t h, int tier, char order, char side, char uplo, long m, long n,
+                      double alpha, const void* A, long lda, const void* B, long ldb, double beta,
+                      void* C, long ldc);
+TB_API int tb_syrk_ex(tb_handle_t h, int tier, char order, char uplo, char trans, long n, long k,
+                      double alpha, const void* A, long lda, double beta, void* C, long ldc,
                       long nb);
-TB_API int tb_symm_ex(tb_handle_t h, int tier, char side, char uplo, long m, long n, double alpha,
-                      const void* A, long lda, const void* B, long ldb, double beta, void* C, long ldc);
-TB_API int tb_syrk_ex(tb_handle_t h, int tier, char uplo, char trans, long n, long k, double alpha,
-                      const void* A, long lda, double beta, void* C, long ldc, long nb);
-TB_API int tb_syr2k_ex(tb_handle_t h, int tier, char uplo, char trans, long n, long k, double alpha,
-                       const void* A, long lda, const void* B, long ldb, double beta, void* C, long ldc,
-                       long nb);
+TB_API int tb_syr2k_ex(tb_handle_t h, int tier, char order, char uplo, char trans, long n, long k,
+                       double alpha, const void* A, long lda, const void* B, long ldb,
+                       double beta, void* C, long ldc, long nb);
 /* replaces tc_symm / tc_cumpsgemm_symm (include/TensorBLAS.h:165-167); the OZAKI and
  * FP64 tiers add the FP64 SYMM the reference does not have */
 TB_API int tb_symm(tb_handle_t h, int tier, long m, long n, double alpha, const void* A, long lda,

@@ -42,6 +42,25 @@ int norm_opt(char c, const char* allowed, char* out)
 
 bool dims_fit(long v) { return v >= 0 && v < (1l << 31); }
 
+// storage order 'C' (column-major) or 'R' (row-major).  A row-major matrix is the
+// column-major storage of its transpose, so a row-major call is the column-major call
+// with the other side and the other triangle, m and n exchanged (TRSM, TRMM, SYMM), and
+// the other trans (SYRK, SYR2K: C's transpose is C): the CBLAS mapping.
+int row_major(char order, char* side, char* uplo, char* trans, long* m, long* n)
+{
+    const char o = (char)std::toupper((unsigned char)order);
+    if (o == 'C') return TB_SUCCESS;
+    if (o != 'R') return TB_ERR_INVALID;
+    auto flip = [](char* c, char a, char b) {
+        if (c) *c = *c == a ? b : a;
+    };
+    flip(side, 'L', 'R');
+    flip(uplo, 'L', 'U');
+    flip(trans, 'N', 'T');
+    if (m && n) std::swap(*m, *n);
+    return TB_SUCCESS;
+}
+
 // carves workspace slot 2 into consecutive buffers
 struct ExWs {
     tb_handle_s* h;
@@ -68,12 +87,13 @@ int zero_fill(tb_handle_s* h, size_t es, void* B, long ldb, long m, long n)
 
 }  // namespace
 
-int trsm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char diag, long m, long n,
+int trsm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, char trans, char diag, long m, long n,
             double alpha, const void* A, long lda, void* B, long ldb, long nb)
 {
     char sd, ul, tr, dg;
     if (!h || tier < kPlainF16 || tier > kMfmaF64 || norm_opt(side, "LR", &sd) || norm_opt(uplo, "LU", &ul) ||
-        norm_opt(trans, "NTC", &tr) || norm_opt(diag, "NU", &dg) || !dims_fit(m) || !dims_fit(n) || nb <= 0)
+        norm_opt(trans, "NTC", &tr) || norm_opt(diag, "NU", &dg) || !dims_fit(m) || !dims_fit(n) || nb <= 0 ||
+        row_major(order, &sd, &ul, nullptr, &m, &n))
         return TB_ERR_INVALID;
     const long na = sd == 'L' ? m : n;                     // order of A
     if (lda < std::max(1l, na) || ldb < std::max(1l, m)) return TB_ERR_INVALID;
@@ -130,12 +150,13 @@ int trsm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char dia
     return TB_SUCCESS;
 }
 
-int trmm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char diag, long m, long n,
+int trmm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, char trans, char diag, long m, long n,
             double alpha, const void* A, long lda, const void* B, long ldb, void* C, long ldc, long nb)
 {
     char sd, ul, tr, dg;
     if (!h || tier < kPlainF16 || tier > kMfmaF64 || norm_opt(side, "LR", &sd) || norm_opt(uplo, "LU", &ul) ||
-        norm_opt(trans, "NTC", &tr) || norm_opt(diag, "NU", &dg) || !dims_fit(m) || !dims_fit(n) || nb <= 0)
+        norm_opt(trans, "NTC", &tr) || norm_opt(diag, "NU", &dg) || !dims_fit(m) || !dims_fit(n) || nb <= 0 ||
+        row_major(order, &sd, &ul, nullptr, &m, &n))
         return TB_ERR_INVALID;
     const long na = sd == 'L' ? m : n;
     if (lda < std::max(1l, na) || ldb < std::max(1l, m) || ldc < std::max(1l, m)) return TB_ERR_INVALID;
@@ -187,12 +208,12 @@ int trmm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char dia
     return xf(h, f64, ws + oC, mm, C, ldc, m, n, back);
 }
 
-int symm_ex(tb_handle_s* h, int tier, char side, char uplo, long m, long n, double alpha, const void* A,
+int symm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, long m, long n, double alpha, const void* A,
             long lda, const void* B, long ldb, double beta, void* C, long ldc)
 {
     char sd, ul;
     if (!h || tier < kPlainF16 || tier > kMfmaF64 || norm_opt(side, "LR", &sd) || norm_opt(uplo, "LU", &ul) ||
-        !dims_fit(m) || !dims_fit(n))
+        !dims_fit(m) || !dims_fit(n) || row_major(order, &sd, &ul, nullptr, &m, &n))
         return TB_ERR_INVALID;
     const long na = sd == 'L' ? m : n;
     if (lda < std::max(1l, na) || ldb < std::max(1l, m) || ldc < std::max(1l, m)) return TB_ERR_INVALID;
@@ -239,12 +260,12 @@ int on_triangle(tb_handle_s* h, bool f64, char ul, long n, void* C, long ldc, si
 
 }  // namespace
 
-int syrk_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+int syrk_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n, long k, double alpha, const void* A,
             long lda, double beta, void* C, long ldc, long nb)
 {
     char ul, tr;
     if (!h || tier < kPlainF16 || tier > kMfmaF64 || norm_opt(uplo, "LU", &ul) || norm_opt(trans, "NTC", &tr) ||
-        !dims_fit(n) || !dims_fit(k) || nb <= 0)
+        !dims_fit(n) || !dims_fit(k) || nb <= 0 || row_major(order, nullptr, &ul, &tr, nullptr, nullptr))
         return TB_ERR_INVALID;
     if (ldc < std::max(1l, n) || lda < std::max(1l, tr == 'N' ? n : k)) return TB_ERR_INVALID;
     if (n == 0) return TB_SUCCESS;
@@ -258,12 +279,12 @@ int syrk_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, dou
     });
 }
 
-int syr2k_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+int syr2k_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n, long k, double alpha, const void* A,
              long lda, const void* B, long ldb, double beta, void* C, long ldc, long nb)
 {
     char ul, tr;
     if (!h || tier < kPlainF16 || tier > kMfmaF64 || norm_opt(uplo, "LU", &ul) || norm_opt(trans, "NTC", &tr) ||
-        !dims_fit(n) || !dims_fit(k) || nb <= 0)
+        !dims_fit(n) || !dims_fit(k) || nb <= 0 || row_major(order, nullptr, &ul, &tr, nullptr, nullptr))
         return TB_ERR_INVALID;
     const long ar = tr == 'N' ? n : k;
     if (ldc < std::max(1l, n) || lda < std::max(1l, ar) || ldb < std::max(1l, ar)) return TB_ERR_INVALID;

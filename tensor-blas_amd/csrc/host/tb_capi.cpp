@@ -154,34 +154,34 @@ int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* 
     return potrf(h, tier, n, A, lda, nb, info);
 }
 
-int tb_trsm_ex(tb_handle_t h, int tier, char side, char uplo, char transa, char diag, long m, long n,
+int tb_trsm_ex(tb_handle_t h, int tier, char order, char side, char uplo, char transa, char diag, long m, long n,
                double alpha, const void* A, long lda, void* B, long ldb, long nb)
 {
-    return trsm_ex(h, tier, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, nb);
+    return trsm_ex(h, tier, order, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, nb);
 }
 
-int tb_trmm_ex(tb_handle_t h, int tier, char side, char uplo, char transa, char diag, long m, long n,
+int tb_trmm_ex(tb_handle_t h, int tier, char order, char side, char uplo, char transa, char diag, long m, long n,
                double alpha, const void* A, long lda, const void* B, long ldb, void* C, long ldc, long nb)
 {
-    return trmm_ex(h, tier, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, C, ldc, nb);
+    return trmm_ex(h, tier, order, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, C, ldc, nb);
 }
 
-int tb_symm_ex(tb_handle_t h, int tier, char side, char uplo, long m, long n, double alpha, const void* A,
+int tb_symm_ex(tb_handle_t h, int tier, char order, char side, char uplo, long m, long n, double alpha, const void* A,
                long lda, const void* B, long ldb, double beta, void* C, long ldc)
 {
-    return symm_ex(h, tier, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+    return symm_ex(h, tier, order, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-int tb_syrk_ex(tb_handle_t h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+int tb_syrk_ex(tb_handle_t h, int tier, char order, char uplo, char trans, long n, long k, double alpha, const void* A,
                long lda, double beta, void* C, long ldc, long nb)
 {
-    return syrk_ex(h, tier, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, nb);
+    return syrk_ex(h, tier, order, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, nb);
 }
 
-int tb_syr2k_ex(tb_handle_t h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+int tb_syr2k_ex(tb_handle_t h, int tier, char order, char uplo, char trans, long n, long k, double alpha, const void* A,
                 long lda, const void* B, long ldb, double beta, void* C, long ldc, long nb)
 {
-    return syr2k_ex(h, tier, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb);
+    return syr2k_ex(h, tier, order, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb);
 }
 
 int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* A, long lda, void* B,

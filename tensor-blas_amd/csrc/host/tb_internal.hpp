@@ -119,15 +119,15 @@ int trmm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, 
 int potrf(tb_handle_s* h, int tier, long n, void* A, long lda, long nb, long* info);
 // BLAS option sets (side 'L'/'R', uplo 'L'/'U', trans 'N'/'T'/'C', diag 'N'/'U'),
 // mapped onto the native forms above by transposed and reversed operand copies
-int trsm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char diag, long m, long n,
+int trsm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, char trans, char diag, long m, long n,
             double alpha, const void* A, long lda, void* B, long ldb, long nb);
-int trmm_ex(tb_handle_s* h, int tier, char side, char uplo, char trans, char diag, long m, long n,
+int trmm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, char trans, char diag, long m, long n,
             double alpha, const void* A, long lda, const void* B, long ldb, void* C, long ldc, long nb);
-int symm_ex(tb_handle_s* h, int tier, char side, char uplo, long m, long n, double alpha, const void* A,
+int symm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, long m, long n, double alpha, const void* A,
             long lda, const void* B, long ldb, double beta, void* C, long ldc);
-int syrk_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+int syrk_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n, long k, double alpha, const void* A,
             long lda, double beta, void* C, long ldc, long nb);
-int syr2k_ex(tb_handle_s* h, int tier, char uplo, char trans, long n, long k, double alpha, const void* A,
+int syr2k_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n, long k, double alpha, const void* A,
              long lda, const void* B, long ldb, double beta, void* C, long ldc, long nb);
 int trsm(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B, long ldb,
          long nb);

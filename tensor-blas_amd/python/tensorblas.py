@@ -29,7 +29,7 @@ class TensorBLASError(RuntimeError):
 
 
 _lib = None
-_L, _D, _P, _I = ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+_L, _D, _P, _I, _C = ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_char
 
 
 def lib():
@@ -51,14 +51,11 @@ def lib():
             "tb_trsm": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
             "tb_trsm_refined": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
             "tb_potrf": [_P, _I, _L, _P, _L, _L, ctypes.POINTER(ctypes.c_long)],
-            "tb_trsm_ex": [_P, _I, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, _L, _L,
-                           _D, _P, _L, _P, _L, _L],
-            "tb_trmm_ex": [_P, _I, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, _L, _L,
-                           _D, _P, _L, _P, _L, _P, _L, _L],
-            "tb_symm_ex": [_P, _I, ctypes.c_char, ctypes.c_char, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
-            "tb_syrk_ex": [_P, _I, ctypes.c_char, ctypes.c_char, _L, _L, _D, _P, _L, _D, _P, _L, _L],
-            "tb_syr2k_ex": [_P, _I, ctypes.c_char, ctypes.c_char, _L, _L, _D, _P, _L, _P, _L, _D, _P,
-                            _L, _L],
+            "tb_trsm_ex": [_P, _I] + [_C] * 5 + [_L, _L, _D, _P, _L, _P, _L, _L],
+            "tb_trmm_ex": [_P, _I] + [_C] * 5 + [_L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
+            "tb_symm_ex": [_P, _I] + [_C] * 3 + [_L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
+            "tb_syrk_ex": [_P, _I] + [_C] * 3 + [_L, _L, _D, _P, _L, _D, _P, _L, _L],
+            "tb_syr2k_ex": [_P, _I] + [_C] * 3 + [_L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _L],
             "tb_symm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
             "tb_gemm": [_P, _I, _I, _I, _L, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
             "tb_s2h": [_P, _L, _L, _P, _L, _P, _L],
@@ -198,26 +195,30 @@ class Handle:
         _check("tb_potrf", lib().tb_potrf(self._h, tier_id(tier), n, A, lda, nb, ctypes.byref(info)))
         return int(info.value)
 
-    # BLAS option sets (characters as in BLAS: side L/R, uplo L/U, trans N/T, diag N/U)
-    def trsm_ex(self, tier, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, nb=256):
-        _check("tb_trsm_ex", lib().tb_trsm_ex(self._h, tier_id(tier), _c(side), _c(uplo), _c(trans),
-                                              _c(diag), m, n, alpha, A, lda, B, ldb, nb))
+    # BLAS option sets (characters as in BLAS: side L/R, uplo L/U, trans N/T, diag N/U;
+    # order "C" column-major or "R" row-major as in CBLAS)
+    def trsm_ex(self, tier, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, nb=256, order="C"):
+        _check("tb_trsm_ex", lib().tb_trsm_ex(self._h, tier_id(tier), _c(order), _c(side), _c(uplo),
+                                              _c(trans), _c(diag), m, n, alpha, A, lda, B, ldb, nb))
 
-    def trmm_ex(self, tier, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, C, ldc, nb=256):
-        _check("tb_trmm_ex", lib().tb_trmm_ex(self._h, tier_id(tier), _c(side), _c(uplo), _c(trans),
-                                              _c(diag), m, n, alpha, A, lda, B, ldb, C, ldc, nb))
+    def trmm_ex(self, tier, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, C, ldc, nb=256,
+                order="C"):
+        _check("tb_trmm_ex", lib().tb_trmm_ex(self._h, tier_id(tier), _c(order), _c(side), _c(uplo),
+                                              _c(trans), _c(diag), m, n, alpha, A, lda, B, ldb, C, ldc,
+                                              nb))
 
-    def symm_ex(self, tier, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
-        _check("tb_symm_ex", lib().tb_symm_ex(self._h, tier_id(tier), _c(side), _c(uplo), m, n, alpha,
-                                              A, lda, B, ldb, beta, C, ldc))
+    def symm_ex(self, tier, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc, order="C"):
+        _check("tb_symm_ex", lib().tb_symm_ex(self._h, tier_id(tier), _c(order), _c(side), _c(uplo), m,
+                                              n, alpha, A, lda, B, ldb, beta, C, ldc))
 
-    def syrk_ex(self, tier, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, nb=256):
-        _check("tb_syrk_ex", lib().tb_syrk_ex(self._h, tier_id(tier), _c(uplo), _c(trans), n, k, alpha,
-                                              A, lda, beta, C, ldc, nb))
+    def syrk_ex(self, tier, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, nb=256, order="C"):
+        _check("tb_syrk_ex", lib().tb_syrk_ex(self._h, tier_id(tier), _c(order), _c(uplo), _c(trans), n,
+                                              k, alpha, A, lda, beta, C, ldc, nb))
 
-    def syr2k_ex(self, tier, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=256):
-        _check("tb_syr2k_ex", lib().tb_syr2k_ex(self._h, tier_id(tier), _c(uplo), _c(trans), n, k,
-                                                alpha, A, lda, B, ldb, beta, C, ldc, nb))
+    def syr2k_ex(self, tier, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=256,
+                 order="C"):
+        _check("tb_syr2k_ex", lib().tb_syr2k_ex(self._h, tier_id(tier), _c(order), _c(uplo), _c(trans),
+                                                n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb))
 
     def symm(self, tier, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
         _check("tb_symm", lib().tb_symm(self._h, tier_id(tier), m, n, alpha, A, lda, B, ldb, beta, C,

@@ -2,7 +2,8 @@
 
 Every option combination of TRSM, TRMM, SYMM, SYRK and SYR2K runs on the device and is
 compared with scipy.linalg.blas (LAPACK's reference-interface dtrsm / dtrmm / dsymm /
-dsyrk / dsyr2k) on the same inputs rounded to the tier's storage type.  The option
+dsyrk / dsyr2k) on the same inputs rounded to the tier's storage type, in column-major
+and in row-major (CBLAS) storage.  The option
 identities (transposition, reversal, unit diagonal) are exact, so the errors are those of
 the native routine: relative Frobenius error within TOL[tier] (fp64 1e-12, Ozaki 1e-11,
 split and fp32 2e-6, plain 5e-3).  The triangle a routine must not read holds NaN on input,
@@ -53,23 +54,37 @@ def clean(T):
     return np.nan_to_num(T).astype(np.float64)
 
 
+def put(M, order, pad):
+    """device copy of M in column-major ("C") or row-major ("R") storage, leading
+    dimension = rows (resp. columns) + pad"""
+    if order == "C":
+        return to_dev(M, M.shape[0] + pad)
+    return to_dev(M.T, M.shape[1] + pad)
+
+
+def get(d, m, n, ld, order):
+    return from_dev(d, m, n, ld) if order == "C" else from_dev(d, n, m, ld).T
+
+
 OPTS4 = list(itertools.product("LR", "LU", "NT", "NU"))
 
 
 @pytest.mark.parametrize("tier", TIERS)
 @pytest.mark.parametrize("side,uplo,trans,diag", OPTS4)
 @pytest.mark.parametrize("m,n", [(37, 70), (300, 45)])
-def test_trsm_ex(h, tier, side, uplo, trans, diag, m, n):
+@pytest.mark.parametrize("order", ["C", "R"])
+def test_trsm_ex(h, order, tier, side, uplo, trans, diag, m, n):
     g = rng(m + 3 * n + ord(side) + 5 * ord(uplo) + 7 * ord(trans) + 11 * ord(diag))
     dt = dtype_of(tier)
     na = m if side == "L" else n
     A = tri(g, na, uplo == "L", dt)
     B = g.uniform(-1, 1, (m, n)).astype(dt)
     alpha = 0.75
-    dA, lda = to_dev(A, na + 2)
-    dB, ldb = to_dev(B, m + 1)
-    h.trsm_ex(tier, side, uplo, trans, diag, m, n, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb)
-    X = from_dev(dB, m, n, ldb)
+    dA, lda = put(A, order, 2)
+    dB, ldb = put(B, order, 1)
+    h.trsm_ex(tier, side, uplo, trans, diag, m, n, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb,
+              order=order)
+    X = get(dB, m, n, ldb, order)
     R = blas.dtrsm(alpha, clean(A), B.astype(np.float64), side=side == "R", lower=uplo == "L",
                    trans_a=trans == "T", diag=diag == "U")
     assert rel(X, R) < TOL[tier], (tier, rel(X, R))
@@ -78,29 +93,31 @@ def test_trsm_ex(h, tier, side, uplo, trans, diag, m, n):
 @pytest.mark.parametrize("tier", TIERS)
 @pytest.mark.parametrize("side,uplo,trans,diag", OPTS4)
 @pytest.mark.parametrize("m,n", [(37, 70), (300, 45)])
-def test_trmm_ex(h, tier, side, uplo, trans, diag, m, n):
+@pytest.mark.parametrize("order", ["C", "R"])
+def test_trmm_ex(h, order, tier, side, uplo, trans, diag, m, n):
     g = rng(7 * m + n + ord(side) + 5 * ord(uplo) + 7 * ord(trans) + 11 * ord(diag))
     dt = dtype_of(tier)
     na = m if side == "L" else n
     A = tri(g, na, uplo == "L", dt)
     B = g.uniform(-1, 1, (m, n)).astype(dt)
     alpha = -1.25
-    dA, lda = to_dev(A, na + 1)
-    dB, ldb = to_dev(B, m + 2)
-    dC, ldc = to_dev(np.zeros((m, n), dt), m + 3)
+    dA, lda = put(A, order, 1)
+    dB, ldb = put(B, order, 2)
+    dC, ldc = put(np.zeros((m, n), dt), order, 3)
     h.trmm_ex(tier, side, uplo, trans, diag, m, n, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb,
-              dC.data_ptr(), ldc)
-    C = from_dev(dC, m, n, ldc)
+              dC.data_ptr(), ldc, order=order)
+    C = get(dC, m, n, ldc, order)
     R = blas.dtrmm(alpha, clean(A), B.astype(np.float64), side=side == "R", lower=uplo == "L",
                    trans_a=trans == "T", diag=diag == "U")
     assert rel(C, R) < TOL[tier], (tier, rel(C, R))
-    assert np.array_equal(from_dev(dB, m, n, ldb), B)            # B is read only
+    assert np.array_equal(get(dB, m, n, ldb, order), B)          # B is read only
 
 
 @pytest.mark.parametrize("tier", TIERS)
 @pytest.mark.parametrize("side,uplo", list(itertools.product("LR", "LU")))
 @pytest.mark.parametrize("m,n,beta", [(37, 70, 0.5), (300, 45, 0.0)])
-def test_symm_ex(h, tier, side, uplo, m, n, beta):
+@pytest.mark.parametrize("order", ["C", "R"])
+def test_symm_ex(h, order, tier, side, uplo, m, n, beta):
     g = rng(3 * m + n + ord(side) + 5 * ord(uplo))
     dt = dtype_of(tier)
     na = m if side == "L" else n
@@ -110,12 +127,12 @@ def test_symm_ex(h, tier, side, uplo, m, n, beta):
     A[np.triu_indices(na, 1) if uplo == "L" else np.tril_indices(na, -1)] = np.nan
     B = g.uniform(-1, 1, (m, n)).astype(dt)
     C0 = g.uniform(-1, 1, (m, n)).astype(dt)
-    dA, lda = to_dev(A, na + 1)
-    dB, ldb = to_dev(B, m + 1)
-    dC, ldc = to_dev(C0 if beta else np.full((m, n), np.nan, dt), m + 2)
+    dA, lda = put(A, order, 1)
+    dB, ldb = put(B, order, 1)
+    dC, ldc = put(C0 if beta else np.full((m, n), np.nan, dt), order, 2)
     h.symm_ex(tier, side, uplo, m, n, 1.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta,
-              dC.data_ptr(), ldc)
-    C = from_dev(dC, m, n, ldc)
+              dC.data_ptr(), ldc, order=order)
+    C = get(dC, m, n, ldc, order)
     R = blas.dsymm(1.5, S.astype(np.float64), B.astype(np.float64), beta, C0.astype(np.float64),
                    side=side == "R", lower=uplo == "L")
     assert rel(C, R) < TOL[tier], (tier, rel(C, R))
@@ -132,15 +149,16 @@ def sym_case(g, n, uplo, dt):
 @pytest.mark.parametrize("tier", TIERS)
 @pytest.mark.parametrize("uplo,trans", list(itertools.product("LU", "NT")))
 @pytest.mark.parametrize("n,k", [(45, 70), (300, 33)])
-def test_syrk_ex(h, tier, uplo, trans, n, k):
+@pytest.mark.parametrize("order", ["C", "R"])
+def test_syrk_ex(h, order, tier, uplo, trans, n, k):
     g = rng(n + 5 * k + ord(uplo) + 3 * ord(trans))
     dt = dtype_of(tier)
     A = g.uniform(-1, 1, (n, k) if trans == "N" else (k, n)).astype(dt)
     C0, off = sym_case(g, n, uplo, dt)
-    dA, lda = to_dev(A, A.shape[0] + 1)
-    dC, ldc = to_dev(C0, n + 2)
-    h.syrk_ex(tier, uplo, trans, n, k, 0.5, dA.data_ptr(), lda, 2.0, dC.data_ptr(), ldc)
-    C = from_dev(dC, n, n, ldc)
+    dA, lda = put(A, order, 1)
+    dC, ldc = put(C0, order, 2)
+    h.syrk_ex(tier, uplo, trans, n, k, 0.5, dA.data_ptr(), lda, 2.0, dC.data_ptr(), ldc, order=order)
+    C = get(dC, n, n, ldc, order)
     R = blas.dsyrk(0.5, A.astype(np.float64), 2.0, np.nan_to_num(C0).astype(np.float64),
                    trans=int(trans == "T"), lower=int(uplo == "L"))
     keep = np.tril if uplo == "L" else np.triu
@@ -151,19 +169,20 @@ def test_syrk_ex(h, tier, uplo, trans, n, k):
 @pytest.mark.parametrize("tier", TIERS)
 @pytest.mark.parametrize("uplo,trans", list(itertools.product("LU", "NT")))
 @pytest.mark.parametrize("n,k", [(45, 70), (300, 33)])
-def test_syr2k_ex(h, tier, uplo, trans, n, k):
+@pytest.mark.parametrize("order", ["C", "R"])
+def test_syr2k_ex(h, order, tier, uplo, trans, n, k):
     g = rng(3 * n + k + ord(uplo) + 5 * ord(trans))
     dt = dtype_of(tier)
     shape = (n, k) if trans == "N" else (k, n)
     A = g.uniform(-1, 1, shape).astype(dt)
     B = g.uniform(-1, 1, shape).astype(dt)
     C0, off = sym_case(g, n, uplo, dt)
-    dA, lda = to_dev(A, shape[0] + 1)
-    dB, ldb = to_dev(B, shape[0] + 3)
-    dC, ldc = to_dev(C0, n + 1)
+    dA, lda = put(A, order, 1)
+    dB, ldb = put(B, order, 3)
+    dC, ldc = put(C0, order, 1)
     h.syr2k_ex(tier, uplo, trans, n, k, -0.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, 1.0,
-               dC.data_ptr(), ldc)
-    C = from_dev(dC, n, n, ldc)
+               dC.data_ptr(), ldc, order=order)
+    C = get(dC, n, n, ldc, order)
     R = blas.dsyr2k(-0.5, A.astype(np.float64), B.astype(np.float64), 1.0,
                     np.nan_to_num(C0).astype(np.float64), trans=int(trans == "T"),
                     lower=int(uplo == "L"))
@@ -212,6 +231,9 @@ def test_options_arguments(h):
         lambda: h.symm_ex("fp64", "L", "L", 4, 4, 1.0, p, 4, p, 3, 0.0, p, 4),  # ldb < m
         lambda: h.syrk_ex("fp64", "U", "T", 4, 8, 1.0, p, 4, 0.0, p, 4),        # lda < k
         lambda: h.syr2k_ex("fp64", "L", "X", 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4),
+        lambda: h.syrk_ex("fp64", "L", "N", 4, 4, 1.0, p, 4, 0.0, p, 4, order="X"),
+        lambda: h.trsm_ex("fp64", "L", "L", "N", "N", 4, 2, 1.0, p, 2, p, 2, order="R"),  # lda < m
+        lambda: h.trsm_ex("fp64", "L", "L", "N", "N", 4, 2, 1.0, p, 4, p, 1, order="R"),  # ldb < n
     ]
     for f in bad:
         with pytest.raises(tensorblas.TensorBLASError):
