@@ -149,6 +149,14 @@ TB_API int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* 
  * Synchronizes the handle's stream to read info.  float* A for plain/split/fp32,
  * double* A for ozaki/fp64; nb is passed to tb_trsm and tb_syrk. */
 TB_API int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* info);
+/* Triangular inverse in place, A := inv(A), for the uplo triangle of A (no counterpart in
+ * the reference; LAPACK dtrtri's contract): uplo 'L'/'U', diag 'N' or 'U' (unit diagonal,
+ * not referenced).  Composed from this library's TRSM in the chosen tier, with FP64
+ * inverses of the diagonal blocks; the other strict triangle is not changed.  *info = 0,
+ * or i when A(i, i) is exactly zero (A is then left unchanged).  n^3/3 flops.
+ * Synchronizes the handle's stream (diag 'N': to read the zero test). */
+TB_API int tb_trtri(tb_handle_t h, int tier, char uplo, char diag, long n, void* A, long lda,
+                    long nb, long* info);
 /* The BLAS option sets around the fixed forms above (no counterpart in the reference,
  * whose routines take one option set each).  Options are BLAS characters: order 'C'
  * (column-major) or 'R' (row-major, as CBLAS: every matrix argument and its leading

@@ -3,6 +3,7 @@
 // the reference's util/util.cu:127-280 and include/TensorBLAS.h:185-284.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "aux_device.hpp"
@@ -42,6 +43,28 @@ trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride
 {
     trtri_block(L + (long long)blockIdx.y * bstride, lda, s, Z + (long long)blockIdx.y * s * s,
                 dinv + (long long)blockIdx.y * s, blockIdx.x * kTrtriCols);
+}
+
+// one column c of the block per workgroup: A(r, c) = dinv[r] (delta(r, c) + W(r, c)),
+// W(r, c) = Z[c + r*s] (Z = W^T)
+template <typename T>
+__global__ void __launch_bounds__(256)
+trtri_store_kernel(const double* __restrict__ Z, const double* __restrict__ dinv, int s,
+                   T* __restrict__ A, long long lda, int unit)
+{
+    const int c = blockIdx.x;
+    for (int r = c + unit + (int)threadIdx.x; r < s; r += 256) {
+        const double w = r == c ? 1.0 : Z[c + (long long)r * s];
+        A[r + (long long)c * lda] = (T)(dinv[r] * w);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+diag_zero_kernel(const T* __restrict__ A, long long lda, long long n, int* __restrict__ info)
+{
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x)
+        if (A[i + i * lda] == T(0)) atomicMin(info, (int)(i + 1));
 }
 
 // FP32 -> FP16, round to nearest even
@@ -229,6 +252,31 @@ hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda
     else
         hipLaunchKernelGGL((trtri_lower_t_kernel<float, float>), g, dim3(64), 0, st,
                            (const float*)L, lda, bstride, s, (float*)Z, dinv);
+    return hipGetLastError();
+}
+
+hipError_t trtri_store(bool f64, const double* Z, const double* dinv, int s, void* A, long long lda,
+                       bool unit, hipStream_t st)
+{
+    if (s <= 0) return hipSuccess;
+    if (f64)
+        hipLaunchKernelGGL(trtri_store_kernel<double>, dim3(s), dim3(256), 0, st, Z, dinv, s,
+                           (double*)A, lda, (int)unit);
+    else
+        hipLaunchKernelGGL(trtri_store_kernel<float>, dim3(s), dim3(256), 0, st, Z, dinv, s,
+                           (float*)A, lda, (int)unit);
+    return hipGetLastError();
+}
+
+hipError_t diag_zero_scan(bool f64, const void* A, long long lda, long long n, int* info,
+                          hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 g((unsigned)std::min<long long>((n + 255) / 256, 1024));
+    if (f64)
+        hipLaunchKernelGGL(diag_zero_kernel<double>, g, dim3(256), 0, st, (const double*)A, lda, n, info);
+    else
+        hipLaunchKernelGGL(diag_zero_kernel<float>, g, dim3(256), 0, st, (const float*)A, lda, n, info);
     return hipGetLastError();
 }
 

@@ -62,7 +62,7 @@ int tb_destroy(tb_handle_t h)
     if (!h) return TB_ERR_INVALID;
     int st = hip_status(hipStreamSynchronize(h->stream));
     prof_clear(h);
-    for (int s = 0; s < 3; ++s)
+    for (int s = 0; s < tb_handle_s::kWsSlots; ++s)
         if (h->ws[s]) (void)hipFree(h->ws[s]);
     if (h->dev_info) (void)hipFree(h->dev_info);
     delete h;
@@ -110,7 +110,13 @@ int tb_set_split_planes(tb_handle_t h, int nplanes)
     return TB_SUCCESS;
 }
 
-long tb_workspace_bytes(tb_handle_t h) { return h ? (long)(h->ws_bytes[0] + h->ws_bytes[1]) : 0; }
+long tb_workspace_bytes(tb_handle_t h)
+{
+    if (!h) return 0;
+    size_t b = 0;
+    for (int s = 0; s < tb_handle_s::kWsSlots; ++s) b += h->ws_bytes[s];
+    return (long)b;
+}
 
 const char* tb_status_string(int status)
 {
@@ -152,6 +158,12 @@ int tb_trsm(tb_handle_t h, int tier, long m, long n, const void* A, long lda, vo
 int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* info)
 {
     return potrf(h, tier, n, A, lda, nb, info);
+}
+
+int tb_trtri(tb_handle_t h, int tier, char uplo, char diag, long n, void* A, long lda, long nb,
+             long* info)
+{
+    return trtri(h, tier, uplo, diag, n, A, lda, nb, info);
 }
 
 int tb_trsm_ex(tb_handle_t h, int tier, char order, char side, char uplo, char transa, char diag, long m, long n,

@@ -28,10 +28,12 @@ struct tb_handle_s {
     int max_cus = 0;        // > 0: GEMM launches occupy at most this many CUs
     // device workspaces, grown on demand, freed by tb_destroy: slot 0 for routines,
     // slot 1 for the Ozaki GEMM (which routines call with their own slot live), slot 2
-    // for the BLAS-option layer's transformed operands (live across a routine call)
-    void* ws[3] = {nullptr, nullptr, nullptr};
-    size_t ws_bytes[3] = {0, 0, 0};
-    int* dev_info = nullptr;   // POTRF status word (device), allocated on first use
+    // for the BLAS-option layer's transformed operands (live across a routine call),
+    // slot 3 for a factorization's own copies (live across option-layer calls)
+    static constexpr int kWsSlots = 4;
+    void* ws[kWsSlots] = {};
+    size_t ws_bytes[kWsSlots] = {};
+    int* dev_info = nullptr;   // POTRF / TRTRI status word (device), allocated on first use
 };
 
 namespace tb {
@@ -117,6 +119,8 @@ int syr2k(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A,
 int trmm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, long lda,
          const void* B, long ldb, void* C, long ldc, long nb);
 int potrf(tb_handle_s* h, int tier, long n, void* A, long lda, long nb, long* info);
+int trtri(tb_handle_s* h, int tier, char uplo, char diag, long n, void* A, long lda, long nb,
+          long* info);
 // BLAS option sets (side 'L'/'R', uplo 'L'/'U', trans 'N'/'T'/'C', diag 'N'/'U'),
 // mapped onto the native forms above by transposed and reversed operand copies
 int trsm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, char trans, char diag, long m, long n,

@@ -1,0 +1,117 @@
+// Triangular inverse in place, A := inv(A) (LAPACK dtrtri's contract), composed from the
+// library's TRSM and the diagonal-block inversion kernel of the TRSM leaves.
+//
+//   lower, order s:  s <= 256: invert the diagonal block on the device in FP64
+//                    (trtri_lower_t, then trtri_store writes D^-1 (I + W))
+//                    else, with A = [L11 . ; L21 L22] split at n1 (a multiple of 64 near
+//                    s/2), inv(A) = [X11 . ; X21 X22] with X21 = -X22 L21 X11, i.e.
+//                    L22 X21 L11 = -L21, solved on the original blocks:
+//                      L21 := -L21 L11^-1      tb_trsm_ex (right, lower, N)
+//                      L21 := L22^-1 L21       tb_trsm_ex (left, lower, N)
+//                    then X11, X22 recursively
+//   upper:           inv(U) = inv(U^T)^T: the lower inverse of a transposed copy
+//                    (workspace slot 3), copied back whole: its upper triangle holds A's
+//                    lower triangle transposed, so that part of A is rewritten unchanged.
+// Every solve runs in the caller's tier.  Work: n^3 / 3 flops.
+// info: LAPACK's, i if A(i, i) is exactly zero (non-unit only); A is then unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <climits>
+
+#include "tb_internal.hpp"
+
+namespace tb {
+
+namespace {
+
+constexpr long kTrtriLeaf = 256;    // largest block trtri_lower_t inverts
+
+struct TrtriCtx {
+    tb_handle_s* h;
+    int tier;
+    char* A;          // byte pointer to A(0, 0) of the lower-triangular matrix
+    long lda;
+    size_t es;        // element size
+    long nb;
+    char diag;
+
+    bool f64() const { return es == 8; }
+    void* at(long i, long j) const { return A + (size_t)(i + j * lda) * es; }
+
+    int leaf(long off, long s)
+    {
+        const bool unit = diag == 'U';
+        const size_t zb = (size_t)s * s * 8, db = (size_t)s * 8;
+        const size_t ob = unit ? ((size_t)s * s * es + 255) / 256 * 256 : 0;
+        char* ws = nullptr;
+        TB_TRY(workspace_get(h, ob + zb + db + 256, &ws, 2));
+        const void* src = at(off, off);
+        long ld = lda;
+        if (unit) {      // a copy with ones on the diagonal: A's diagonal is not referenced
+            TB_HIP(xform(f64(), src, lda, ws, s, (int)s, (int)s, kXformUnitDiag, 1.0, h->stream));
+            src = ws;
+            ld = s;
+        }
+        double* Z = reinterpret_cast<double*>(ws + ob);
+        double* dinv = reinterpret_cast<double*>(ws + ob + zb);
+        hipEvent_t e0;
+        TB_TRY(prof_begin(h, &e0));
+        TB_HIP(trtri_lower_t(f64(), true, src, ld, 0, (int)s, 1, Z, dinv, h->stream));
+        TB_HIP(trtri_store(f64(), Z, dinv, (int)s, at(off, off), lda, unit, h->stream));
+        return prof_end(h, kProfTrtri, e0, (double)s * s * s / 3.0);
+    }
+
+    int rec(long off, long s)
+    {
+        if (s <= kTrtriLeaf) return leaf(off, s);
+        const long n1 = std::min(s - 1, (s / 2 + 63) / 64 * 64);
+        const long n2 = s - n1;
+        void* L21 = at(off + n1, off);
+        TB_TRY(trsm_ex(h, tier, 'C', 'R', 'L', 'N', diag, n2, n1, -1.0, at(off, off), lda, L21, lda, nb));
+        TB_TRY(trsm_ex(h, tier, 'C', 'L', 'L', 'N', diag, n2, n1, 1.0, at(off + n1, off + n1), lda, L21,
+                       lda, nb));
+        TB_TRY(rec(off, n1));
+        return rec(off + n1, n2);
+    }
+};
+
+}  // namespace
+
+int trtri(tb_handle_s* h, int tier, char uplo, char diag, long n, void* A, long lda, long nb, long* info)
+{
+    const char ul = (char)std::toupper((unsigned char)uplo), dg = (char)std::toupper((unsigned char)diag);
+    if (!h || !info || tier < kPlainF16 || tier > kMfmaF64 || (ul != 'L' && ul != 'U') ||
+        (dg != 'N' && dg != 'U') || n < 0 || n >= (1l << 31) || nb <= 0 || lda < std::max(1l, n) ||
+        (n > 0 && !A))
+        return TB_ERR_INVALID;
+    *info = 0;
+    if (n == 0) return TB_SUCCESS;
+    const bool f64 = (tier == kOzakiI8 || tier == kMfmaF64);
+    const size_t es = f64 ? 8 : 4;
+    if (dg == 'N') {     // singular: report the first zero on the diagonal, change nothing
+        if (!h->dev_info) TB_HIP(hipMalloc(&h->dev_info, sizeof(int)));
+        TB_HIP(fill_i32(h->dev_info, 1, INT_MAX, h->stream));
+        TB_HIP(diag_zero_scan(f64, A, lda, n, h->dev_info, h->stream));
+        int v = 0;
+        TB_HIP(hipMemcpyAsync(&v, h->dev_info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        TB_HIP(hipStreamSynchronize(h->stream));
+        if (v != INT_MAX) {
+            *info = v;
+            return TB_SUCCESS;
+        }
+    }
+    if (ul == 'L') {
+        TrtriCtx c{h, tier, static_cast<char*>(A), lda, es, nb, dg};
+        return c.rec(0, n);
+    }
+    char* T = nullptr;
+    TB_TRY(workspace_get(h, (size_t)n * n * es, &T, 3));
+    TB_HIP(xform(f64, A, lda, T, n, (int)n, (int)n, kXformTrans, 1.0, h->stream));
+    TrtriCtx c{h, tier, T, n, es, nb, dg};
+    TB_TRY(c.rec(0, n));
+    return hip_status(xform(f64, T, n, A, lda, (int)n, (int)n, kXformTrans, 1.0, h->stream));
+}
+
+}  // namespace tb

@@ -74,6 +74,14 @@ constexpr int kTrtriCols = 8;
 hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda,
                          long long bstride, int s, int nblocks, void* Z, double* dinv,
                          hipStream_t st);
+// Writes inv(L_q) = D^-1 (I + W) into the lower triangle of the s x s block A (ld lda)
+// from trtri_lower_t's FP64 outputs (one block: Z s x s, dinv s); unit: the diagonal of
+// A is left as it is.  The strict upper triangle is not written.
+hipError_t trtri_store(bool f64, const double* Z, const double* dinv, int s, void* A, long long lda,
+                       bool unit, hipStream_t st);
+// *info = min(*info, i + 1) over the i < n with A(i, i) == 0
+hipError_t diag_zero_scan(bool f64, const void* A, long long lda, long long n, int* info,
+                          hipStream_t st);
 
 // ------------------------------------------------------------- conversions etc.
 // ah(i,j) = RNE fp16(as(i,j)), m x n (util/util.cu:127-135 s2h; include/TensorBLAS.h:185-284)

@@ -9,7 +9,7 @@ every tier:
   reference routines' fixed option sets, returning a new tensor.  ``tc_syrk`` and
   ``tc_trsm`` take a ``tier`` too; the default "plain" keeps the reference's call.
 * ``syrk``, ``syr2k``, ``trmm``, ``trsm``, ``symm`` take the BLAS options (side, uplo,
-  trans, diag, alpha, beta) and ``potrf`` factors in place.
+  trans, diag, alpha, beta); ``potrf`` factors and ``trtri`` inverts in place.
 
 Tensors are 2-D CUDA tensors, float32 for the "plain", "split" and "fp32" tiers and
 float64 for "fp64" and "ozaki".  A tensor may be row-major (torch's default) or a
@@ -264,3 +264,18 @@ def potrf(A, upper=False, tier="split", nb=256):
         A.copy_(At)
         return info
     return _handle(A).potrf(tier, n, A.data_ptr(), lda, nb=nb)
+
+
+def trtri(A, upper=False, unit=False, tier="split", nb=256):
+    """Triangular inverse in place: the lower (upper=False) or upper triangle of A is
+    replaced by that of inv(A); the other strict triangle (and, with unit=True, the
+    diagonal) is neither read nor written.  Returns LAPACK's info (0, or i for a zero
+    A(i, i), A then unchanged); synchronizes the stream."""
+    _check(tier, A)
+    n = A.shape[0]
+    _dims_square(A, n, "A")
+    order, lda = _out_layout(A)
+    # a row-major triangle is the other triangle of the column-major memory, and
+    # inv(A^T) = inv(A)^T, so the call is the column-major one with uplo exchanged
+    uplo = "U" if upper == (order == "C") else "L"
+    return _handle(A).trtri(tier, uplo, "U" if unit else "N", n, A.data_ptr(), lda, nb=nb)

@@ -51,6 +51,7 @@ def lib():
             "tb_trsm": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
             "tb_trsm_refined": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
             "tb_potrf": [_P, _I, _L, _P, _L, _L, ctypes.POINTER(ctypes.c_long)],
+            "tb_trtri": [_P, _I, _C, _C, _L, _P, _L, _L, ctypes.POINTER(ctypes.c_long)],
             "tb_trsm_ex": [_P, _I] + [_C] * 5 + [_L, _L, _D, _P, _L, _P, _L, _L],
             "tb_trmm_ex": [_P, _I] + [_C] * 5 + [_L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
             "tb_symm_ex": [_P, _I] + [_C] * 3 + [_L, _L, _D, _P, _L, _P, _L, _D, _P, _L],
@@ -193,6 +194,14 @@ class Handle:
         leading minor of order j is not positive definite).  Synchronizes."""
         info = ctypes.c_long(-1)
         _check("tb_potrf", lib().tb_potrf(self._h, tier_id(tier), n, A, lda, nb, ctypes.byref(info)))
+        return int(info.value)
+
+    def trtri(self, tier, uplo, diag, n, A, lda, nb=256):
+        """A := inv(A) for the uplo triangle; returns LAPACK's info (0, or i for a zero
+        A(i, i), A unchanged)."""
+        info = ctypes.c_long(0)
+        _check("tb_trtri", lib().tb_trtri(self._h, tier_id(tier), _c(uplo), _c(diag), n, A, lda, nb,
+                                          ctypes.byref(info)))
         return int(info.value)
 
     # BLAS option sets (characters as in BLAS: side L/R, uplo L/U, trans N/T, diag N/U;
