@@ -25,6 +25,8 @@ Other workloads (--workload):
          checked equal to the isolated calls
   potrf  recursive Cholesky n = 16384 (split tier against FP32-FMA and plain), error
          against host FP64 LAPACK dpotrf, host OpenBLAS spotrf baseline
+  trtri  triangular inverse n = 16384 (split against FP32-FMA and plain), error against
+         host FP64 LAPACK dtrtri, host OpenBLAS strtri baseline
   gemm   tb_gemm microbenchmark (--m --n --k --tiers)
   copy   a 256 MiB device copy (reference for HBM byte counters)
 
@@ -357,6 +359,69 @@ def run_potrf(args, torch, tensorblas):
         line["cpu_baseline"] = {"value": round(flops / t / 1e12, 4), "unit": "TFLOP/s",
                                 "cores": int(host_threads()), "kind": "port",
                                 "impl": "host OpenBLAS spotrf via scipy.linalg.lapack",
+                                "sample": f"full problem, one run ({t:.2f} s)"}
+    h.close()
+    return line
+
+
+# --------------------------------------------------------- triangular inverse
+def run_trtri(args, torch, tensorblas):
+    """In-place inverse of the TRSM benchmark's lower-triangular L (strictly lower
+    U[-1,1], diagonal n + U[0,1]), split tier against the FP32-FMA and plain tiers;
+    error against the host FP64 LAPACK inverse."""
+    n, nb = args.n or 16384, args.nb
+    g = np.random.default_rng(0xC0F9)
+    L = (np.tril(g.uniform(-1.0, 1.0, (n, n)), -1) + np.diag(n + g.uniform(0, 1, n))).astype(np.float32)
+    dA0 = torch.from_numpy(np.ascontiguousarray(L.T)).cuda()     # column-major
+    dA = dA0.clone()
+    h = tensorblas.Handle()
+    h.set_split_planes(args.split_planes)
+    flops = float(n) ** 3 / 3.0
+    infos = []
+
+    def step(tier):
+        def f():
+            dA.copy_(dA0)
+            infos.append(h.trtri(tier, "L", "N", n, dA.data_ptr(), n, nb=nb))
+        return f
+    log("bench: trtri host FP64 reference")
+    import scipy.linalg.lapack as sl
+    R, rinfo = sl.dtrtri(np.asfortranarray(L.astype(np.float64)), lower=1)
+    R = np.tril(R)
+    out, errs = {}, {}
+    for tier in ("split", "fp32", "plain"):
+        out[tier] = timed(h, step(tier), args.steps if tier == "split" else 3, args.warmup)
+        X = np.tril(dA.cpu().numpy().reshape(n, n).T).astype(np.float64)
+        d = np.abs(X - R)
+        errs[tier] = {"normwise": float(np.linalg.norm(d) / np.linalg.norm(R)),
+                      "maxrel": float(d.max() / np.abs(R).max())}
+    assert rinfo == 0 and all(i == 0 for i in infos), (rinfo, set(infos))
+    prof = profiled(h, step("split"))
+    roof, rk = roofline({k_: v for k_, v in prof.items() if k_.startswith("gemm_")}, FP16_PEAK_TF,
+                        "TFLOP/s", "mfma", f"trtri{n}",
+                        "split tier (2 FP16 planes): 3 FP16 MFMA products per algorithmic flop")
+    ms = out["split"]
+    line = line_base("eff. TFLOP/s of the triangular inverse at FP32-level error (n^3/3 flops)",
+                     flops / (ms * 1e-3) / 1e12, "TFLOP/s", args.steps, args.warmup, ms,
+                     f"f32 ({args.split_planes}-plane fp16 split MFMA)",
+                     "synthetic: L strictly lower U[-1,1], diagonal n + U[0,1] (seeded)",
+                     {"workload": "TRTRI inv(L) lower in place, recursive (TRMM + FP64 diagonal blocks)",
+                      "n": n, "nb": nb})
+    line.update({
+        "tiers_tflops": {t: round(flops / (v * 1e-3) / 1e12, 2) for t, v in out.items()},
+        "split_vs_fp32_fma": round(out["fp32"] / out["split"], 3),
+        "errors_vs_fp64": errs, "roofline": roof, "roofline_kernel": rk,
+        "profile_ms": {k_: round(v[1], 3) for k_, v in prof.items()},
+        "note": "each step restores L (device copy) before the inversion",
+    })
+    if not args.no_cpu:
+        Lf = np.asfortranarray(L)
+        t0 = time.perf_counter()
+        sl.strtri(Lf, lower=1)
+        t = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(flops / t / 1e12, 4), "unit": "TFLOP/s",
+                                "cores": int(host_threads()), "kind": "port",
+                                "impl": "host OpenBLAS strtri via scipy.linalg.lapack",
                                 "sample": f"full problem, one run ({t:.2f} s)"}
     h.close()
     return line
@@ -795,7 +860,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="trsm",
-                    choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched", "serve", "potrf"])
+                    choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched", "serve", "potrf",
+                             "trtri"])
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=32.0)
     ap.add_argument("--load", type=float, default=0.7)
@@ -832,7 +898,7 @@ def main():
     import tensorblas
     run = {"trsm": run_trsm, "syrk": run_syrk, "ozaki": run_ozaki, "gemm": run_gemm,
            "copy": run_copy, "batched": run_batched, "serve": run_serve,
-           "potrf": run_potrf}[args.workload]
+           "potrf": run_potrf, "trtri": run_trtri}[args.workload]
     print(json.dumps(run(args, torch, tensorblas)), flush=True)
 
 

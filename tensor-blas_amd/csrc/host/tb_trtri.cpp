@@ -4,15 +4,15 @@
 //   lower, order s:  s <= 256: invert the diagonal block on the device in FP64
 //                    (trtri_lower_t, then trtri_store writes D^-1 (I + W))
 //                    else, with A = [L11 . ; L21 L22] split at n1 (a multiple of 64 near
-//                    s/2), inv(A) = [X11 . ; X21 X22] with X21 = -X22 L21 X11, i.e.
-//                    L22 X21 L11 = -L21, solved on the original blocks:
-//                      L21 := -L21 L11^-1      tb_trsm_ex (right, lower, N)
-//                      L21 := L22^-1 L21       tb_trsm_ex (left, lower, N)
-//                    then X11, X22 recursively
+//                    s/2), inv(A) = [X11 . ; X21 X22] with X21 = -X22 L21 X11:
+//                      X11, X22 recursively, in place
+//                      T := L21 X11            tb_trmm_ex (right, lower, N) into workspace
+//                      L21 := -X22 T           tb_trmm (left, lower, N: the native form)
 //   upper:           inv(U) = inv(U^T)^T: the lower inverse of a transposed copy
 //                    (workspace slot 3), copied back whole: its upper triangle holds A's
 //                    lower triangle transposed, so that part of A is rewritten unchanged.
-// Every solve runs in the caller's tier.  Work: n^3 / 3 flops.
+// The products are single triangular GEMM launches (no inner solves), in the caller's
+// tier.  Work: n^3 / 3 flops.
 // info: LAPACK's, i if A(i, i) is exactly zero (non-unit only); A is then unchanged.
 #include <hip/hip_runtime.h>
 
@@ -36,6 +36,7 @@ struct TrtriCtx {
     size_t es;        // element size
     long nb;
     char diag;
+    char* T;          // n2 x n1 product buffer (ld n2) in workspace slot 3
 
     bool f64() const { return es == 8; }
     void* at(long i, long j) const { return A + (size_t)(i + j * lda) * es; }
@@ -68,12 +69,13 @@ struct TrtriCtx {
         if (s <= kTrtriLeaf) return leaf(off, s);
         const long n1 = std::min(s - 1, (s / 2 + 63) / 64 * 64);
         const long n2 = s - n1;
-        void* L21 = at(off + n1, off);
-        TB_TRY(trsm_ex(h, tier, 'C', 'R', 'L', 'N', diag, n2, n1, -1.0, at(off, off), lda, L21, lda, nb));
-        TB_TRY(trsm_ex(h, tier, 'C', 'L', 'L', 'N', diag, n2, n1, 1.0, at(off + n1, off + n1), lda, L21,
-                       lda, nb));
         TB_TRY(rec(off, n1));
-        return rec(off + n1, n2);
+        TB_TRY(rec(off + n1, n2));
+        void* L21 = at(off + n1, off);
+        TB_TRY(trmm_ex(h, tier, 'C', 'R', 'L', 'N', diag, n2, n1, 1.0, at(off, off), lda, L21, lda, T, n2,
+                       nb));
+        return trmm_ex(h, tier, 'C', 'L', 'L', 'N', diag, n2, n1, -1.0, at(off + n1, off + n1), lda, T, n2,
+                       L21, lda, nb);
     }
 };
 
@@ -102,16 +104,19 @@ int trtri(tb_handle_s* h, int tier, char uplo, char diag, long n, void* A, long 
             return TB_SUCCESS;
         }
     }
+    // slot 3: [transposed copy of A (upper only)] [product buffer, <= n/2 x (n/2 + 64)]
+    const size_t cb = ul == 'U' ? ((size_t)n * n * es + 255) / 256 * 256 : 0;
+    const size_t tb = n > kTrtriLeaf ? (size_t)(n / 2 + 64) * (n / 2 + 64) * es : 0;
+    char* ws = nullptr;
+    TB_TRY(workspace_get(h, cb + tb + 256, &ws, 3));
     if (ul == 'L') {
-        TrtriCtx c{h, tier, static_cast<char*>(A), lda, es, nb, dg};
+        TrtriCtx c{h, tier, static_cast<char*>(A), lda, es, nb, dg, ws};
         return c.rec(0, n);
     }
-    char* T = nullptr;
-    TB_TRY(workspace_get(h, (size_t)n * n * es, &T, 3));
-    TB_HIP(xform(f64, A, lda, T, n, (int)n, (int)n, kXformTrans, 1.0, h->stream));
-    TrtriCtx c{h, tier, T, n, es, nb, dg};
+    TB_HIP(xform(f64, A, lda, ws, n, (int)n, (int)n, kXformTrans, 1.0, h->stream));
+    TrtriCtx c{h, tier, ws, n, es, nb, dg, ws + cb};
     TB_TRY(c.rec(0, n));
-    return hip_status(xform(f64, T, n, A, lda, (int)n, (int)n, kXformTrans, 1.0, h->stream));
+    return hip_status(xform(f64, ws, n, A, lda, (int)n, (int)n, kXformTrans, 1.0, h->stream));
 }
 
 }  // namespace tb
