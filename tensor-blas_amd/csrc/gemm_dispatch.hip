@@ -28,16 +28,22 @@ hipError_t run16(const GemmLaunch& L, hipStream_t s, int max_cus)
 template <int NPL, int SA, int SB>
 hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s, int mc)
 {
-    // four waves; a stage is (BM + BN) x 64 B per plane, and the ring holds as many
-    // stages (<= 4) as fit in the 160 KB of LDS.  Square and leaf tiles: 64 x 64
-    // accumulators per wave.  Wide tiles: 128 x 64 per wave, which reads 25 % fewer LDS
-    // bytes per MFMA; at 2 planes its two accumulator sets fill the 256 AGPRs.
+    // a stage is (BM + BN) x 64 B per plane, and the ring holds as many stages (<= 4)
+    // as fit in the 160 KB of LDS.  Eight waves (two per SIMD) where the registers
+    // allow.  Square, leaf and 2-plane wide tiles: 64 x 64 accumulators per wave.  The
+    // plain wide tile, 256 x 256, has 128 x 64 per wave: 25 % fewer LDS bytes per MFMA
+    // (+14 % at 8192^3 over 256 x 128); at 2 planes that tile's two accumulator sets
+    // would not fit two waves per SIMD, and four 128 x 64 waves on 256 x 128 measured
+    // no faster than eight 64 x 64 ones.
     if (shape == kShapeLeaf) {
         if constexpr (NPL == 2) return run16<NPL, SA, SB, 64, 256, 1, 8, 3>(L, s, mc);
         return run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s, mc);
     }
     if constexpr (NPL <= 2 && SA == kSrcPlanes && SB == kSrcPlanes) {
-        if (shape == kShapeWide) return run16<NPL, SA, SB, 256, 128, 4, 2, NPL == 2 ? 3 : 4>(L, s, 0);
+        if (shape == kShapeWide) {
+            if constexpr (NPL == 1) return run16<NPL, SA, SB, 256, 256, 2, 4, 4>(L, s, 0);
+            return run16<NPL, SA, SB, 256, 128, 4, 2, 3>(L, s, 0);
+        }
     }
     if (shape == kShapeWide) return hipErrorInvalidValue;
     return run16<NPL, SA, SB, 128, 128, 2, 4, NPL == 3 ? 3 : 4>(L, s, mc);
@@ -69,7 +75,7 @@ int capped_grid(const void* kernel, int threads, int total, int max_cus)
 TileShape gemm_tile_shape(int tier, int shape)
 {
     if (shape == kShapeLeaf) return TileShape{64, 256};
-    if (shape == kShapeWide) return TileShape{256, 128};
+    if (shape == kShapeWide) return tier == kPlainF16 ? TileShape{256, 256} : TileShape{256, 128};
     return TileShape{128, 128};
 }
 

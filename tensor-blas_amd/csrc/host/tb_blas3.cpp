@@ -126,8 +126,8 @@ int GemmQueue::add(const GemmProb& p0)
 }
 
 // Square launches of full-C plane products with at least one wide tile per CU run on
-// 256 x 128 tiles instead (fewer LDS bytes per MFMA); smaller launches keep 128 x 128,
-// whose twice as many tiles fill more of the device.
+// wide tiles instead (256 x 256 plain, 256 x 128 split: fewer operand bytes per MFMA);
+// smaller launches keep 128 x 128, whose more numerous tiles fill more of the device.
 bool GemmQueue::retile_wide()
 {
     const int npl = planes_of(h_, tier_);

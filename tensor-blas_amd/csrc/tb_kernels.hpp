@@ -14,9 +14,9 @@ enum GemmShape : int {
     kShapeSquare = 0,  // 128 x 128 tiles: off-diagonal updates and whole routines
     kShapeLeaf = 1,    // 64 x 256 tiles: a workgroup owns all n <= 256 columns of its rows,
                        // so C may alias op(A) (in-place TRSM leaf X := B * inv(L)^T)
-    kShapeWide = 2,    // 256 x 128 tiles (128 x 64 per wave): full C, both operands in
-                       // FP16 planes, plain or 2-plane split; chosen by GemmQueue::flush
-                       // for launches with enough tiles to fill the device
+    kShapeWide = 2,    // 256 x 256 tiles (plain) or 256 x 128 (2-plane split): full C,
+                       // both operands in FP16 planes; chosen by GemmQueue::flush for
+                       // launches with enough tiles to fill the device
 };
 struct TileShape {
     int bm, bn;
