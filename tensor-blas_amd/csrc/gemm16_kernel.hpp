@@ -467,9 +467,15 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
             Frags f0, f1;
             load_frags(s, 0, f0);
             load_frags(s, 16, f1);
+            // split tier: the wave in its MFMA block takes issue priority over the SIMD's
+            // other wave (+2-3 % at 8192^3; the plain tier measured 3 % slower with it)
+            if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
             mma(f0);
+            if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
             if (ahead) issue_dma(s + NST - 1);
+            if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
             mma(f1);
+            if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
             if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32)
                 if (more) store_regs(s + 1);
             if (KEEP > 0 && ahead) TB_WAIT_VM_BARRIER(KEEP);
