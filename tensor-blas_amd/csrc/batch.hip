@@ -20,7 +20,7 @@ namespace tb {
 
 namespace {
 
-constexpr int kPP = kSrcPlanes, kFF = kSrcF32;
+constexpr int kPP = kSrcPlanes;
 
 // stages of the plane-sourced panel / square tiles: as many as fit in the LDS
 template <int NPL>
@@ -30,22 +30,28 @@ constexpr int sq_stages() { return NPL == 3 ? 3 : 4; }
 template <int NPL, int SRC>
 constexpr int panel_nst() { return SRC == kSrcPlanes ? panel_stages<NPL>() : 2; }
 
-template <int NPL, int SRC>
-using CfgPanel = Gemm16Cfg<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, 4, panel_nst<NPL, SRC>()>;
-template <int NPL>
-using CfgSq = Gemm16Cfg<NPL, kSrcPlanes, kSrcPlanes, 128, 128, 2, 2, sq_stages<NPL>()>;
-template <int NPL, int SRC>
-using LdsPanel = Gemm16Lds<CfgPanel<NPL, SRC>, kPanelRows, kPanelCols>;
-template <int NPL>
-using LdsSq = Gemm16Lds<CfgSq<NPL>, 128, 128>;
+// waves per workgroup: eight (two per SIMD, as the single calls' split tiles) when every
+// tile is 2-plane; 64-row panels of 1 or 3 planes do not split into whole direct-to-LDS
+// instructions over eight waves, so those plans keep four
+template <int UNPL, int ANPL>
+constexpr int batch_waves() { return UNPL == 2 && ANPL == 2 ? 8 : 4; }
+
+template <int NPL, int SRC, int NW>
+using CfgPanel = Gemm16Cfg<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, NW, panel_nst<NPL, SRC>()>;
+template <int NPL, int NW>
+using CfgSq = Gemm16Cfg<NPL, kSrcPlanes, kSrcPlanes, 128, 128, 2, NW / 2, sq_stages<NPL>()>;
+template <int NPL, int SRC, int NW>
+using LdsPanel = Gemm16Lds<CfgPanel<NPL, SRC, NW>, kPanelRows, kPanelCols>;
+template <int NPL, int NW>
+using LdsSq = Gemm16Lds<CfgSq<NPL, NW>, 128, 128>;
 
 constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 
-template <int NPL, int SRC>
+template <int NPL, int SRC, int NW>
 __device__ __forceinline__ void panel_tile(const GemmProb& P, int i0, int j0, char* raw)
 {
-    auto* L = reinterpret_cast<LdsPanel<NPL, SRC>*>(raw);
-    gemm16_tile<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, 4, panel_nst<NPL, SRC>()>(
+    auto* L = reinterpret_cast<LdsPanel<NPL, SRC, NW>*>(raw);
+    gemm16_tile<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, NW, panel_nst<NPL, SRC>()>(
         P, 0, i0, j0, L->ring, L->scA, L->scB, L->emax);
 }
 
@@ -59,14 +65,18 @@ __device__ __forceinline__ int load_exp(const int* p)
 // planes at dst (plane p, row r, column k at dst[p*sp + r*ldp + k]) with columns
 // K .. round_up(K, 64) and rows >= m zero.  Split planes are scaled per row by
 // 2^(15 - e[row]) exactly as split_planes_block converts them.  t: a 64 x 65 LDS tile.
-template <int NPL>
+// NT threads (256 or 512): four per row, each converting 16 (NT = 256) or 8 columns of
+// every 64-column block.
+template <int NPL, int NT>
 __device__ __forceinline__ void panel_planes(const float* __restrict__ X, long long ld, int m, int i0, int K,
                              const int* __restrict__ e, _Float16* __restrict__ dst, long long ldp,
                              long long sp, float (*t)[65])
 {
     typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    static_assert(NT == 256 || NT == 512, "threads");
+    constexpr int kHalves = NT == 256 ? 2 : 1;       // 8-column halves per thread
     const int tid = threadIdx.x;
-    const int rl = tid >> 2, cl0 = (tid & 3) * 16;
+    const int rl = (tid >> 2) & 63, cl0 = (tid & 3) * 16, h0 = NT == 256 ? 0 : tid >> 8;
     int sc = 0;
     if (NPL >= 2 && i0 + rl < m) {
         const int ex = e[i0 + rl];
@@ -75,13 +85,14 @@ __device__ __forceinline__ void panel_planes(const float* __restrict__ X, long l
     const int kend = (K + 63) / 64 * 64;
     for (int c0 = 0; c0 < kend; c0 += 64) {
         __syncthreads();                                 // t is free
-        for (int q = tid; q < 64 * 64; q += 256) {
+        for (int q = tid; q < 64 * 64; q += NT) {
             const int rr = q & 63, cc = q >> 6;
             const int r = i0 + rr, k = c0 + cc;
             t[rr][cc] = (r < m && k < K) ? X[(long long)r + (long long)k * ld] : 0.0f;
         }
         __syncthreads();
-        for (int half = 0; half < 2; ++half) {
+        for (int hq = 0; hq < kHalves; ++hq) {
+            const int half = h0 + hq;
             half8 hh, mm, ll;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -112,7 +123,7 @@ __device__ __forceinline__ void panel_planes(const float* __restrict__ X, long l
 }
 
 // rows i0 .. i0+63 of one TRSM instance through its whole schedule
-template <int UNPL, int ANPL>
+template <int UNPL, int ANPL, int NW>
 __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* __restrict__ steps, int i0,
                            const BatchScratch& S, char* raw)
 {
@@ -141,8 +152,8 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
         if (st.type == 0) {
             // leaf: B(:, off:off+s) := (B + B W^T) D^-1 in place (one tile spans all s columns)
             const int off = st.a, s = st.b;
-            panel_planes<ANPL>(I.C + (long long)off * I.ldc, I.ldc, I.m, i0, s, I.eSnap, scr, S.ldp,
-                               S.sp, t);
+            panel_planes<ANPL, 64 * NW>(I.C + (long long)off * I.ldc, I.ldc, I.m, i0, s, I.eSnap, scr,
+                                        S.ldp, S.sp, t);
             P.n = s;
             P.k = s;
             P.A = scrA;
@@ -156,7 +167,7 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
             P.alpha = 1.0;
             P.cscale = I.dinv + off;
             P.eB = I.eZ + st.eoff;
-            panel_tile<ANPL, kPP>(P, i0, 0, raw);
+            panel_tile<ANPL, kPP, NW>(P, i0, 0, raw);
             continue;
         }
         // update: B(:, c0:c0+w) -= B(:, k0:k0+kw) * L(c0:c0+w, k0:k0+kw)^T, in the update
@@ -179,30 +190,30 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
             P.ldb = I.ldpL;
             P.srcA = P.srcB = kPP;
             if (st.type == 1) {
-                panel_planes<UNPL>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap, scr,
-                                   S.ldp, S.sp, t);
+                panel_planes<UNPL, 64 * NW>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap,
+                                            scr, S.ldp, S.sp, t);
                 P.B = static_cast<const _Float16*>(sh ? I.LpU1 : I.LpU) + kb;
                 P.spB = I.spLU;
-                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<UNPL, kPP>(P, i0, j0, raw);
+                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<UNPL, kPP, NW>(P, i0, j0, raw);
             } else {
-                panel_planes<ANPL>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap, scr,
-                                   S.ldp, S.sp, t);
+                panel_planes<ANPL, 64 * NW>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap,
+                                            scr, S.ldp, S.sp, t);
                 P.B = static_cast<const _Float16*>(sh ? I.LpA1 : I.LpA) + kb;
                 P.spB = I.spLA;
-                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<ANPL, kPP>(P, i0, j0, raw);
+                for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<ANPL, kPP, NW>(P, i0, j0, raw);
             }
         }
     }
 }
 
 template <int UNPL, int ANPL, int SQ>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((64 * batch_waves<UNPL, ANPL>()))
 batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* __restrict__ insts,
              const BatchStep* __restrict__ steps, int* counter, BatchScratch S)
 {
-    constexpr size_t kPanels = cmax(cmax(sizeof(LdsPanel<UNPL, kPP>), sizeof(LdsPanel<ANPL, kPP>)),
-                                    cmax(sizeof(LdsPanel<UNPL, kFF>), sizeof(LdsPanel<ANPL, kFF>)));
-    constexpr size_t kLds = cmax(cmax(kPanels, SQ ? sizeof(LdsSq<UNPL>) : 0), 64 * 65 * sizeof(float));
+    constexpr int NW = batch_waves<UNPL, ANPL>();
+    constexpr size_t kPanels = cmax(sizeof(LdsPanel<UNPL, kPP, NW>), sizeof(LdsPanel<ANPL, kPP, NW>));
+    constexpr size_t kLds = cmax(cmax(kPanels, SQ ? sizeof(LdsSq<UNPL, NW>) : 0), 64 * 65 * sizeof(float));
     __shared__ __attribute__((aligned(1024))) char raw[kLds];
     __shared__ int next_s;
     for (;;) {
@@ -214,7 +225,7 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
         const BatchItem w = items[it];
         const BatchInst& I = insts[w.inst];
         if (I.op == kBatchTrsm) {
-            trsm_panel<UNPL, ANPL>(I, steps, w.i0, S, raw);
+            trsm_panel<UNPL, ANPL, NW>(I, steps, w.i0, S, raw);
         } else {
             GemmProb P = {};
             P.m = P.n = I.n;
@@ -232,11 +243,11 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
             P.eA = P.eB = I.eRow;
             P.batch = 1;
             if constexpr (SQ) {
-                auto* L = reinterpret_cast<LdsSq<UNPL>*>(raw);
-                gemm16_tile<UNPL, kPP, kPP, 128, 128, 2, 2, sq_stages<UNPL>()>(
+                auto* L = reinterpret_cast<LdsSq<UNPL, NW>*>(raw);
+                gemm16_tile<UNPL, kPP, kPP, 128, 128, 2, NW / 2, sq_stages<UNPL>()>(
                     P, 0, w.i0, w.j0, L->ring, L->scA, L->scB, L->emax);
             } else {
-                panel_tile<UNPL, kPP>(P, w.i0, w.j0, raw);
+                panel_tile<UNPL, kPP, NW>(P, w.i0, w.j0, raw);
             }
         }
     }
@@ -295,12 +306,13 @@ hipError_t run_sq(int sq, const BatchItem* items, int nitems, const BatchInst* i
                   const BatchStep* steps, int* counter, const BatchScratch& S, int grid,
                   hipStream_t s)
 {
+    const dim3 block(64 * batch_waves<UNPL, ANPL>());
     if (sq)
-        hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 1>), dim3(grid), dim3(256), 0, s, items, nitems,
-                           insts, steps, counter, S);
+        hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 1>), dim3(grid), block, 0, s, items, nitems, insts,
+                           steps, counter, S);
     else
-        hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 0>), dim3(grid), dim3(256), 0, s, items, nitems,
-                           insts, steps, counter, S);
+        hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 0>), dim3(grid), block, 0, s, items, nitems, insts,
+                           steps, counter, S);
     return hipGetLastError();
 }
 
