@@ -762,7 +762,7 @@ def run_serve(args, torch, tensorblas):
     arr -= arr[0]
     res = tensorblas.serve(slots, budget, [dict(r, arrival_us=float(a)) for r, a in zip(reqs, arr)],
                            large_us=args.large_ms * 1e3, max_large=args.max_large, order=args.order,
-                           reserve_cus=args.reserve_cus)
+                           reserve_cus=args.reserve_cus, partition=args.partition)
     lat = [x["done_us"] - a for x, a in zip(res, arr)]
     ratio = [l_ / i_ for l_, i_ in zip(lat, iso)]
     mix_sum = [checksum(torch, C) for _, _, C, _ in keep]
@@ -783,7 +783,7 @@ def run_serve(args, torch, tensorblas):
                       "slots": nslots, "hw_queues": args.hw_queues,
                       "hbm_budget_gib": args.budget_gib, "load": args.load,
                       "large_ms": args.large_ms, "max_large": args.max_large, "order": args.order,
-                      "reserve_cus": args.reserve_cus,
+                      "reserve_cus": args.reserve_cus, "partition": args.partition,
                       "total_flops": total_flops})
     line.update({
         "latency_ms": {"p50": round(pct(lat, 50) / 1e3, 3), "p99": round(pct(lat, 99) / 1e3, 3)},
@@ -869,6 +869,9 @@ def main():
     ap.add_argument("--max-large", type=int, default=1)
     ap.add_argument("--order", default="shortest", choices=["fifo", "shortest"])
     ap.add_argument("--reserve-cus", type=int, default=0)
+    ap.add_argument("--partition", type=int, default=0, choices=[0, 1, 2],
+                    help="serve: --reserve-cus CUs left free of large requests (0: GEMM CU caps, "
+                         "1: CU-masked streams) or of small requests (2: CU-masked streams)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="serve: HIP hardware queues of the process (default: one per slot, <= 32)")
     ap.add_argument("--count", type=int, default=0)
@@ -891,7 +894,8 @@ def main():
         # One hardware queue per slot stream, set before HIP starts.  With fewer queues
         # than slots HIP maps several slot streams onto one queue, and a small request's
         # kernels then wait in FIFO order behind a large request's on another slot.
-        args.hw_queues = min(32, max(1, args.hw_queues or args.slots))
+        extra = {0: 0, 1: args.max_large, 2: args.slots}[args.partition]
+        args.hw_queues = min(32, max(1, args.hw_queues or args.slots + extra))
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
 
     import torch

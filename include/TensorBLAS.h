@@ -274,9 +274,14 @@ typedef struct {
     int max_large;         /* large requests running at once (>= 1) */
     int order;             /* within a lane: TB_SERVE_FIFO (oldest waiting request first) or
                               TB_SERVE_SHORTEST (shortest estimated time alone first) */
-    int reserve_cus;       /* compute units large requests leave free (their GEMMs run
-                              under tb_set_max_cus), so that small requests always find
-                              free CUs; 0: none */
+    int reserve_cus;       /* compute units large requests leave free, so that small
+                              requests always find free CUs; 0: none */
+    int partition;         /* how they are left free: 0, large requests' GEMMs run under
+                              tb_set_max_cus; 1, large requests run on CU-masked streams
+                              (every kernel; needs GPU_MAX_HW_QUEUES >= nslots + max_large);
+                              2, the other way round: small requests run on CU-masked
+                              streams and the reserved CUs stay free for large ones
+                              (GPU_MAX_HW_QUEUES >= 2 nslots) */
 } tb_serve_config;
 enum { TB_SERVE_FIFO = 0, TB_SERVE_SHORTEST = 1 };
 TB_API int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const tb_request* reqs,

@@ -16,11 +16,21 @@
 // A kernel whose workgroups cannot all be resident keeps the CUs it is given as they
 // free up, so kernels from other streams start only once it has dispatched its last
 // workgroup: a small request admitted next to a large one would wait for most of it.
-// With cfg->reserve_cus > 0 the GEMMs of large requests are capped (tb_set_max_cus) at
-// (CUs - reserve_cus) / max_large CUs; their workgroups then all fit and loop over the
-// tiles, and the reserved CUs stay free for the small lane.
+// With cfg->reserve_cus > 0 large requests leave that many CUs to the small lane:
+//   partition 0: their GEMMs are capped (tb_set_max_cus) at (CUs - reserve_cus) /
+//     max_large CUs; the workgroups then all fit and loop over the tiles;
+//   partition 1: each runs on a stream created with a CU mask without the reserved
+//     CUs (hipExtStreamCreateWithCUMask), so none of its kernels -- GEMMs, conversions,
+//     slicing, inversions -- ever occupies them.  One such stream per concurrent large
+//     request, swapped in as the slot's stream for that request; each needs a hardware
+//     queue of its own (GPU_MAX_HW_QUEUES >= slots + max_large).
+// partition 2 turns the reservation around: small requests run on masked streams (one
+// per slot; GPU_MAX_HW_QUEUES >= 2 slots) without the reserved CUs, which stay free for
+// the large lane.  Latency targets are absolute, so the longest requests have the least
+// slack: this keeps small requests from slowing them down.
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <thread>
 #include <vector>
 
@@ -115,6 +125,7 @@ struct Running {
     int slot;
     size_t bytes;
     bool large;
+    int mstream;      // index of the masked stream the request runs on, or -1
 };
 
 }  // namespace
@@ -141,17 +152,20 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
 {
     if (!slots || nslots <= 0 || !cfg || cfg->hbm_budget <= 0 || cfg->large_us < 0 ||
         cfg->max_large < 1 || (cfg->order != TB_SERVE_FIFO && cfg->order != TB_SERVE_SHORTEST) ||
-        cfg->reserve_cus < 0 || count < 0 || (count > 0 && (!reqs || !out)))
+        cfg->reserve_cus < 0 || cfg->partition < 0 || cfg->partition > 2 || count < 0 ||
+        (count > 0 && (!reqs || !out)))
         return TB_ERR_INVALID;
     int large_cus = 0;                  // CU cap of a large request's GEMMs (0: none)
+    int cus = 0;
     if (cfg->reserve_cus > 0) {
-        int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, slots[0]->device) !=
             hipSuccess)
             return TB_ERR_DEVICE;
         if (cfg->reserve_cus >= cus) return TB_ERR_INVALID;
-        large_cus = std::max(1, (cus - cfg->reserve_cus) / cfg->max_large);
+        if (cfg->partition == 0) large_cus = std::max(1, (cus - cfg->reserve_cus) / cfg->max_large);
     }
+    // lane whose requests run on masked streams (-1: none)
+    const int masked_lane = cfg->reserve_cus > 0 && cfg->partition > 0 ? (cfg->partition == 1 ? 1 : 0) : -1;
     const size_t budget = (size_t)cfg->hbm_budget;
     for (int s = 0; s < nslots; ++s)
         if (!slots[s]) return TB_ERR_INVALID;
@@ -180,6 +194,27 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
     std::vector<hipEvent_t> done(nslots, nullptr);
     hipEvent_t t0 = nullptr;
     int st = TB_SUCCESS;
+    // masked streams (CUs reserve_cus .. cus-1) of the masked lane: max_large for the
+    // large lane, one per slot for the small lane
+    std::vector<hipStream_t> mstream;
+    std::vector<int> mfree;
+    std::vector<hipStream_t> own(nslots);
+    for (int s = 0; s < nslots; ++s) own[s] = slots[s]->stream;
+    if (masked_lane >= 0) {
+        std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
+        for (int c = cfg->reserve_cus; c < cus; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
+        const int nm = masked_lane == 1 ? cfg->max_large : nslots;
+        for (int q = 0; q < nm && st == TB_SUCCESS; ++q) {
+            hipStream_t m = nullptr;
+            if (hipExtStreamCreateWithCUMask(&m, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+                (void)hipGetLastError();
+                st = TB_ERR_DEVICE;
+                break;
+            }
+            mstream.push_back(m);
+            mfree.push_back(q);
+        }
+    }
     for (int s = 0; s < nslots && st == TB_SUCCESS; ++s)
         if (hipEventCreate(&done[s]) != hipSuccess) st = TB_ERR_DEVICE;
     if (st == TB_SUCCESS && hipEventCreate(&t0) != hipSuccess) st = TB_ERR_DEVICE;
@@ -215,6 +250,10 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
             out[run[q].req].done_us = 1e3 * (double)ms;
             used -= run[q].bytes;
             large_running -= run[q].large ? 1 : 0;
+            if (run[q].mstream >= 0) {               // the slot's own stream again
+                slots[run[q].slot]->stream = own[run[q].slot];
+                mfree.push_back(run[q].mstream);
+            }
             idle.push_back(run[q].slot);
             run[q] = run.back();
             run.pop_back();
@@ -252,20 +291,28 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
                 out[r].slot = s;
                 const int keep_cus = slots[s]->max_cus;
                 if (L == 1 && large_cus > 0) slots[s]->max_cus = large_cus;
+                int ms = -1;
+                if (L == masked_lane) {        // a stream per request the lane may run
+                    ms = mfree.back();
+                    mfree.pop_back();
+                    slots[s]->stream = mstream[(size_t)ms];
+                }
                 out[r].status = launch(slots[s], reqs[r]);
                 slots[s]->max_cus = keep_cus;
                 if (hipEventRecord(done[s], slots[s]->stream) != hipSuccess) st = TB_ERR_DEVICE;
                 used += need[r];
                 large_running += L;
-                run.push_back({r, s, need[r], L == 1});
+                run.push_back({r, s, need[r], L == 1, ms});
                 waiting[L].erase(waiting[L].begin() + (long)w);
                 progress = true;
             }
         }
         if (!progress) std::this_thread::yield();
     }
-    // drain whatever is still running after an error
+    // drain whatever is still running after an error; every slot gets its own stream back
     for (int s = 0; s < nslots; ++s) (void)hipStreamSynchronize(slots[s]->stream);
+    for (int s = 0; s < nslots; ++s) slots[s]->stream = own[s];
+    for (hipStream_t m : mstream) (void)hipStreamDestroy(m);
     for (hipEvent_t e : done)
         if (e) (void)hipEventDestroy(e);
     if (t0) (void)hipEventDestroy(t0);

@@ -111,13 +111,16 @@ def test_serve_rejects_bad_arguments(tbm):
     with pytest.raises(tbm.TensorBLASError):
         tbm.serve(hs, 1 << 20, [dict(op="syrk", tier="plain", n=1, k=1, A=z.data_ptr(), lda=1,
                                      C=z.data_ptr(), ldc=1, arrival_us=-1.0)])
+    with pytest.raises(tbm.TensorBLASError):
+        tbm.serve(hs, 1 << 20, [], reserve_cus=32, partition=3)
 
 
-@pytest.mark.parametrize("reserve_cus", [0, 32])
-def test_serve_lanes(tbm, reserve_cus):
+@pytest.mark.parametrize("reserve_cus,partition", [(0, 0), (32, 0), (32, 1), (32, 2)])
+def test_serve_lanes(tbm, reserve_cus, partition):
     """Requests estimated above large_us run at most max_large at a time; small ones
-    keep flowing (with reserve_cus, on CUs the large requests' GEMMs leave free); every
-    result still equals the isolated one."""
+    keep flowing (with reserve_cus, on CUs the large requests leave free: partition 0
+    caps their GEMMs, 1 runs them on a CU-masked stream; partition 2 runs the small ones
+    on CU-masked streams instead); every result still equals the isolated one."""
     reqs, keep = make_requests(9, sizes=(128, 448))
     alone, est = [], []
     h1 = slots(tbm, 1)
@@ -126,9 +129,11 @@ def test_serve_lanes(tbm, reserve_cus):
         alone.append(C.clone())
     restore(keep)
     thr = float(np.median(est))                # the library's own estimates split the set
-    hs = slots(tbm, 6)
+    # partition 2 masks a stream per slot: each takes a hardware queue, of which a
+    # process has 4 unless GPU_MAX_HW_QUEUES says otherwise
+    hs = slots(tbm, 3 if partition == 2 else 6)
     res = tbm.serve(hs, 1 << 34, [dict(r, arrival_us=0.0) for r in reqs], large_us=thr, max_large=1,
-                    reserve_cus=reserve_cus)
+                    reserve_cus=reserve_cus, partition=partition)
     large = [x for x in res if x["est_us"] > thr]
     assert 0 < len(large) < len(res)
     large.sort(key=lambda x: x["admit_us"])
