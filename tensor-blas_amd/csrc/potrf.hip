@@ -1,17 +1,21 @@
 // Diagonal-block Cholesky of the recursive POTRF (host/tb_potrf.cpp).
 //
-// One workgroup of 1024 threads factors one s x s diagonal block (s <= kPotrfLeaf =
-// 128) in place in FP64, A_jj = L_jj L_jj^T, lower triangle only, right-looking with
-// the block in registers: thread (rt, ct) holds the 4 x 4 tile of rows 4rt.. and
-// columns 4ct.. (tiles above the diagonal stay idle).  Step j reads the pivot column j
-// from a double-buffered LDS column, forms l = a / sqrt(a_jj) for the tile's rows and
-// columns, finalizes column j and subtracts l_i l_k from every later element; the
-// threads holding column j + 1 then publish it.  One barrier per step; 4 waves per
-// SIMD hide the LDS and FP64 latencies.
-// A non-positive (or NaN) pivot at column j stops the factorization and records
-// off + j + 1 in *info if it is still 0 (blocks run in order, so the first failure in
-// the matrix is kept): LAPACK dpotrf's info convention.  A block whose *info is
-// already set returns at once, so the factor stays the partial one.
+// One workgroup factors one s x s diagonal block (s <= kPotrfLeaf = 128) in place in
+// FP64, A_jj = L_jj L_jj^T, lower triangle only, right-looking with the block in
+// registers.  Thread tiles are 4 x 4; a wave holds an 8 x 8 block of tiles (32 x 32
+// elements), and the ten waves hold the ten lower-triangular such blocks, so a wave
+// whose rows or columns are all finished skips a step as a whole.
+// Each step finalizes two columns j, j+1 behind one barrier: every thread reads both
+// (current) columns from a double-buffered LDS pair, forms
+//   d0 = sqrt(a_jj), l_i0 = a_ij / d0, d1 = sqrt(a_j+1,j+1 - l_j+1,0^2),
+//   l_i1 = (a_i,j+1 - l_i0 l_j+1,0) / d1
+// for its rows and columns -- the operations the one-column recursion does, in the
+// same order -- finalizes columns j, j+1 and subtracts l_i0 l_k0, then l_i1 l_k1 from
+// every later element; the threads holding columns j+2, j+3 then publish them.
+// A non-positive (or NaN) pivot at column c stops the factorization and records
+// off + c + 1 in *info if it is still 0 (blocks run in order, so the first failure in
+// the matrix is kept): LAPACK dpotrf's info convention; the columns before c are
+// factored.  A block whose *info is already set returns at once.
 #include <hip/hip_runtime.h>
 
 #include "aux_device.hpp"
@@ -21,17 +25,22 @@ namespace tb {
 
 namespace {
 
-static_assert(kPotrfLeaf == 128, "32 x 32 tiles of 4 x 4");
+static_assert(kPotrfLeaf == 128, "4 x 4 blocks of 8 x 8 tiles of 4 x 4");
+constexpr int kLeafWaves = 10;                   // lower-triangular 32 x 32 blocks
 
 template <typename T>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(64 * kLeafWaves)
 potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* __restrict__ info)
 {
-    __shared__ double col[2][kPotrfLeaf];         // pivot column of the current step
+    __shared__ double colA[2][kPotrfLeaf];        // column j of the current step
+    __shared__ double colB[2][kPotrfLeaf];        // column j + 1
     const int tid = threadIdx.x;
     if (*info != 0) return;                       // an earlier block failed
-    const int ct = tid & 31, rt = tid >> 5;
-    const int i0 = 4 * rt, k0 = 4 * ct;
+    // wave w -> lower block (bi, bj): w = bi (bi + 1) / 2 + bj
+    const int w = tid >> 6, lane = tid & 63;
+    const int bi = w >= 6 ? 3 : (w >= 3 ? 2 : (w >= 1 ? 1 : 0));
+    const int bj = w - bi * (bi + 1) / 2;
+    const int i0 = 32 * bi + 4 * (lane >> 3), k0 = 32 * bj + 4 * (lane & 7);
     const bool live = k0 <= i0 + 3 && i0 < s && k0 < s;   // tile touches the lower block
     double a[4][4];
 #pragma unroll
@@ -43,49 +52,82 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
             a[ii][kk] = masked_load(A + ic + (long long)kc * lda, live && i < s && k <= i);
         }
     }
-    if (ct == 0) {                                // column 0 for step 0
+    if (k0 == 0) {                                // columns 0 and 1 for step 0
 #pragma unroll
-        for (int ii = 0; ii < 4; ++ii) col[0][i0 + ii] = a[ii][0];
+        for (int ii = 0; ii < 4; ++ii) {
+            colA[0][i0 + ii] = a[ii][0];
+            colB[0][i0 + ii] = a[ii][1];
+        }
     }
-    for (int j = 0; j < s; ++j) {
+    for (int j = 0; j < s; j += 2) {
         __syncthreads();
-        const double* cj = col[j & 1];
-        const double piv = cj[j];
-        if (!(piv > 0.0)) {                       // uniform: every thread reads the same pivot
+        const int b = (j >> 1) & 1;
+        const double* ca = colA[b];
+        const double* cb = colB[b];
+        const bool has1 = j + 1 < s;              // uniform
+        const double piv0 = ca[j];
+        if (!(piv0 > 0.0)) {                      // uniform: every thread reads the same pivot
             if (tid == 0) atomicCAS(info, 0, (int)(off + j + 1));
             break;
         }
-        if (!live || i0 + 3 < j) continue;       // every row of the tile finished (no barrier
-                                                  // skipped: the loop's barrier is at its top)
-        const double d = __builtin_sqrt(piv);
-        const double rcp = 1.0 / d;
-        double li[4], lk[4];
+        const double d0 = __builtin_sqrt(piv0);
+        const double r0 = 1.0 / d0;
+        const double l1 = has1 ? ca[j + 1] * r0 : 0.0;
+        const double piv1 = has1 ? cb[j + 1] - l1 * l1 : 1.0;
+        const bool ok1 = piv1 > 0.0;              // uniform; false for NaN
+        const double d1 = __builtin_sqrt(ok1 ? piv1 : 1.0);
+        const double r1 = 1.0 / d1;
+        const bool two = has1 && ok1;             // columns j and j+1 this step
+        if (live && i0 + 3 >= j && k0 + 3 >= j) {   // rows or columns left to finish
+            double li0[4], li1[4], lk0[4], lk1[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            li[q] = cj[i0 + q] * rcp;
-            lk[q] = cj[k0 + q] * rcp;
-        }
+            for (int q = 0; q < 4; ++q) {
+                li0[q] = ca[i0 + q] * r0;
+                li1[q] = (cb[i0 + q] - li0[q] * l1) * r1;
+                lk0[q] = ca[k0 + q] * r0;
+                lk1[q] = (cb[k0 + q] - lk0[q] * l1) * r1;
+            }
 #pragma unroll
-        for (int ii = 0; ii < 4; ++ii) {
-            const int i = i0 + ii;
+            for (int ii = 0; ii < 4; ++ii) {
+                const int i = i0 + ii;
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const int k = k0 + kk;
-                const double upd = a[ii][kk] - li[ii] * lk[kk];
-                const double fin = i == j ? d : (i > j ? li[ii] : a[ii][kk]);
-                a[ii][kk] = k == j ? fin : ((k > j && i > j) ? upd : a[ii][kk]);
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int k = k0 + kk;
+                    double v = a[ii][kk];
+                    if (k == j) {
+                        v = i == j ? d0 : (i > j ? li0[ii] : v);
+                    } else if (two && k == j + 1) {
+                        v = i == j + 1 ? d1 : (i > j + 1 ? li1[ii] : v);
+                    } else if (two && k > j + 1 && i > j + 1) {
+                        v = v - li0[ii] * lk0[kk];
+                        v = v - li1[ii] * lk1[kk];
+                    }
+                    a[ii][kk] = v;
+                }
+            }
+            // the tiles holding columns j+2, j+3 publish them (static indices only: a
+            // select on a run-time column index would put the tile in scratch memory)
+            if (two) {
+                double* na = colA[b ^ 1];
+                double* nb = colB[b ^ 1];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    if (k0 + kk == j + 2) {
+#pragma unroll
+                        for (int ii = 0; ii < 4; ++ii)
+                            if (i0 + ii >= j + 2) na[i0 + ii] = a[ii][kk];
+                    }
+                    if (k0 + kk == j + 3) {
+#pragma unroll
+                        for (int ii = 0; ii < 4; ++ii)
+                            if (i0 + ii >= j + 3) nb[i0 + ii] = a[ii][kk];
+                    }
+                }
             }
         }
-        // the tile holding column j + 1 publishes it (static indices only: a select
-        // on a run-time column index would put the tile in scratch memory)
-        double* cn = col[(j + 1) & 1];
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            if (k0 + kk == j + 1) {
-#pragma unroll
-                for (int ii = 0; ii < 4; ++ii)
-                    if (i0 + ii > j) cn[i0 + ii] = a[ii][kk];
-            }
+        if (has1 && !ok1) {                       // column j is final; j+1 is not positive
+            if (tid == 0) atomicCAS(info, 0, (int)(off + j + 2));
+            break;
         }
     }
 #pragma unroll
@@ -107,10 +149,10 @@ hipError_t potrf_leaf(bool f64, void* A, long long lda, int s, long long off, in
     if (s <= 0) return hipSuccess;
     if (s > kPotrfLeaf) return hipErrorInvalidValue;
     if (f64)
-        hipLaunchKernelGGL(potrf_leaf_kernel<double>, dim3(1), dim3(1024), 0, st,
+        hipLaunchKernelGGL(potrf_leaf_kernel<double>, dim3(1), dim3(64 * kLeafWaves), 0, st,
                            static_cast<double*>(A), lda, s, off, info);
     else
-        hipLaunchKernelGGL(potrf_leaf_kernel<float>, dim3(1), dim3(1024), 0, st,
+        hipLaunchKernelGGL(potrf_leaf_kernel<float>, dim3(1), dim3(64 * kLeafWaves), 0, st,
                            static_cast<float*>(A), lda, s, off, info);
     return hipGetLastError();
 }
