@@ -887,10 +887,13 @@ def main():
     ap.add_argument("--op", default="", choices=["", "syr2k", "symm"])
     ap.add_argument("--split-planes", type=int, default=2, choices=[2, 3])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="trsm: skip the request-stream summary the default line carries")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1")
-    if args.workload == "serve":
+    with_serve = args.workload == "trsm" and not args.no_serve
+    if args.workload == "serve" or with_serve:
         # One hardware queue per slot stream, set before HIP starts.  With fewer queues
         # than slots HIP maps several slot streams onto one queue, and a small request's
         # kernels then wait in FIFO order behind a large request's on another slot.
@@ -903,7 +906,23 @@ def main():
     run = {"trsm": run_trsm, "syrk": run_syrk, "ozaki": run_ozaki, "gemm": run_gemm,
            "copy": run_copy, "batched": run_batched, "serve": run_serve,
            "potrf": run_potrf, "trtri": run_trtri}[args.workload]
-    print(json.dumps(run(args, torch, tensorblas)), flush=True)
+    line = run(args, torch, tensorblas)
+    if with_serve:
+        # the metric's second half: the request stream (--workload serve, default
+        # policy) in brief, measured after the TRSM in the same process
+        torch.cuda.empty_cache()
+        sv = run_serve(args, torch, tensorblas)
+        line["served"] = {
+            "workload": sv["config"]["workload"],
+            "latency_ms": sv["latency_ms"], "isolated_latency_ms": sv["isolated_latency_ms"],
+            "p99_latency_vs_isolated_p99": sv["p99_latency_vs_isolated_p99"],
+            "throughput_tflops": round(sv["value"], 3),
+            "throughput_vs_offline": sv["throughput_vs_offline"],
+            "results_equal_alone": sv["results_equal_alone"],
+            "policy": {k: sv["config"][k] for k in ("slots", "load", "large_ms", "max_large", "order",
+                                                   "reserve_cus", "partition")},
+        }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":
