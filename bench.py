@@ -864,7 +864,7 @@ def main():
                              "trtri"])
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=32.0)
-    ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--load", type=float, default=0.72)
     ap.add_argument("--large-ms", type=float, default=10.0)
     ap.add_argument("--max-large", type=int, default=1)
     ap.add_argument("--order", default="shortest", choices=["fifo", "shortest"])
