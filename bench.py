@@ -862,7 +862,7 @@ def main():
     ap.add_argument("--workload", default="trsm",
                     choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched", "serve", "potrf",
                              "trtri"])
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gib", type=float, default=32.0)
     ap.add_argument("--load", type=float, default=0.72)
     ap.add_argument("--large-ms", type=float, default=10.0)
