@@ -45,7 +45,17 @@ __device__ __forceinline__ void rowexp_mn_block(const T* __restrict__ X, long lo
     int k1 = min(K, k0 + kRowChunk);
     if (lower_only) k1 = min(k1, r + 1);
     int e = kExpZero;
-    for (int k = k0; k < k1; ++k) e = max(e, expo(X[(long long)r + (long long)k * ld]));
+    int k = k0;
+    // eight independent loads in flight per thread: one at a time, the pass waited a
+    // full memory latency per element
+    for (; k + 8 <= k1; k += 8) {
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = X[(long long)r + (long long)(k + q) * ld];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) e = max(e, expo(v[q]));
+    }
+    for (; k < k1; ++k) e = max(e, expo(X[(long long)r + (long long)k * ld]));
     if (e != kExpZero) atomicMax(out + r, e);
 }
 
@@ -63,7 +73,12 @@ __device__ __forceinline__ void rowexp_k_block(const T* __restrict__ X, long lon
     const int k1 = min(K, k0 + kColChunk);
     if (kfrom_diag) k0 = max(k0, r + 1);
     int e = kExpZero;
-    for (int k = k0 + lane; k < k1; k += 64) e = max(e, expo(Xr[k]));
+    int k = k0 + lane;
+    for (; k + 192 < k1; k += 256) {          // four independent loads in flight
+        const T v0 = Xr[k], v1 = Xr[k + 64], v2 = Xr[k + 128], v3 = Xr[k + 192];
+        e = max(max(e, expo(v0)), max(expo(v1), max(expo(v2), expo(v3))));
+    }
+    for (; k < k1; k += 64) e = max(e, expo(Xr[k]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
     if (lane == 0 && e != kExpZero) atomicMax(out + r, e);
