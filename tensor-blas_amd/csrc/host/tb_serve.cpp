@@ -78,8 +78,9 @@ double req_flops(const tb_request& r)
 
 // Time alone on MI355X, t = a + b n + flops / c, per routine and tier (plain, split,
 // Ozaki): a covers launches and conversions, b the dependent chain of a recursion
-// (TRSM's n/nb leaves), c the rate of its products.  Fitted to isolated calls of
-// every routine and tier at n = 128 .. 8192 (k = n, m = n); median error 5 %.
+// (TRSM's n/nb leaves), c the rate of its products.  Fitted (non-negative least
+// squares on relative error) to isolated calls of every routine and tier at n = 128 ..
+// 8192 (k = n, m = n); median error 3-6 %, TRSM 12-16 %.
 // The FP32 and FP64 comparator tiers use a fixed cost and a flat rate.
 struct TimeModel {
     double a_us, b_us_per_n, c_tflops;
@@ -87,11 +88,11 @@ struct TimeModel {
 double est_us(const tb_request& r)
 {
     static const TimeModel model[5][3] = {
-        /* SYRK  */ {{32.0, 0.007, 640.0}, {137.0, 0.015, 269.0}, {174.0, 0.034, 46.0}},
-        /* SYR2K */ {{49.0, 0.002, 613.0}, {272.0, 0.013, 248.0}, {343.0, 0.0, 41.5}},
-        /* TRMM  */ {{37.0, 0.015, 383.0}, {153.0, 0.014, 143.0}, {146.0, 0.031, 25.2}},
-        /* TRSM  */ {{620.0, 0.513, 242.0}, {597.0, 0.708, 191.0}, {43.5, 1.351, 45.7}},
-        /* SYMM  */ {{32.0, 0.024, 715.0}, {149.0, 0.021, 268.0}, {137.0, 0.044, 48.1}},
+        /* SYRK  */ {{30.4, 0.0038, 614.0}, {51.9, 0.0141, 270.0}, {67.6, 0.0297, 44.2}},
+        /* SYR2K */ {{40.8, 0.0061, 614.0}, {92.8, 0.0181, 251.0}, {103.2, 0.104, 46.5}},
+        /* TRMM  */ {{38.0, 0.0090, 425.0}, {67.2, 0.0128, 152.0}, {73.7, 0.0085, 23.5}},
+        /* TRSM  */ {{105.7, 0.452, 1e9}, {99.0, 0.527, 397.0}, {0.0, 0.984, 63.7}},
+        /* SYMM  */ {{37.6, 0.0083, 674.0}, {68.1, 0.0149, 270.0}, {77.8, 0.0074, 44.0}},
     };
     const double f = req_flops(r);
     if (r.tier == TB_TIER_FP32) return 50.0 + f / (70.0 * 1e6);
