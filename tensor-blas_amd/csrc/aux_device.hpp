@@ -219,19 +219,31 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
 {
     const int r0 = by * 64, c0 = bx * 64;   // c: plane column = kshift + k
     const int tid = threadIdx.x;
-    for (int q = tid; q < 64 * 64; q += 256) {
-        int rl, cl;
-        if (mode == kOpT) { cl = q % 64; rl = q / 64; }
-        else { rl = q % 64; cl = q / 64; }
+    // the 16 elements of this thread: all loads issued before the first LDS store
+    // (unconditional loads at a clamped address, then masked: loads behind per-element
+    // branches went out one memory latency apart)
+    float v[16];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int q = tid + it * 256;
+        const int rl = mode == kOpT ? q / 64 : q % 64, cl = mode == kOpT ? q % 64 : q / 64;
         const int r = r0 + rl, k = c0 + cl - kshift;
-        float v = 0.0f;
-        if (r < rows && k >= 0 && k < K) {
-            if (mode == kOpN) v = X[(long long)r + (long long)k * ld];
-            else if (mode == kOpT) v = X[(long long)k + (long long)r * ld];
-            else if (r >= k) v = X[(long long)r + (long long)k * ld];
-            else if (mode == kOpSymL) v = X[(long long)k + (long long)r * ld];
+        bool keep = r < rows && k >= 0 && k < K;
+        long long idx;
+        if (mode == kOpN) {
+            idx = (long long)r + (long long)k * ld;
+        } else if (mode == kOpT) {
+            idx = (long long)k + (long long)r * ld;
+        } else {                               // lower-stored: the other triangle mirrored
+            keep = keep && (r >= k || mode == kOpSymL);
+            idx = r >= k ? (long long)r + (long long)k * ld : (long long)k + (long long)r * ld;
         }
-        t[rl][cl] = v;
+        v[it] = (float)masked_load(X + (keep ? idx : 0ll), keep);
+    }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int q = tid + it * 256;
+        t[mode == kOpT ? q / 64 : q % 64][mode == kOpT ? q % 64 : q / 64] = v[it];
     }
     __syncthreads();
     const int rl = tid >> 2, cl0 = (tid & 3) * 16;
