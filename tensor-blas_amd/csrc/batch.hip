@@ -85,10 +85,20 @@ __device__ __forceinline__ void panel_planes(const float* __restrict__ X, long l
     const int kend = (K + 63) / 64 * 64;
     for (int c0 = 0; c0 < kend; c0 += 64) {
         __syncthreads();                                 // t is free
-        for (int q = tid; q < 64 * 64; q += NT) {
-            const int rr = q & 63, cc = q >> 6;
-            const int r = i0 + rr, k = c0 + cc;
-            t[rr][cc] = (r < m && k < K) ? X[(long long)r + (long long)k * ld] : 0.0f;
+        // every load of the block in flight before the LDS stores (clamped, masked)
+        constexpr int kPer = 64 * 64 / NT;
+        float v[kPer];
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int q = tid + it * NT;
+            const int r = i0 + (q & 63), k = c0 + (q >> 6);
+            const bool keep = r < m && k < K;
+            v[it] = (float)masked_load(X + (keep ? (long long)r + (long long)k * ld : 0ll), keep);
+        }
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int q = tid + it * NT;
+            t[q & 63][q >> 6] = v[it];
         }
         __syncthreads();
         for (int hq = 0; hq < kHalves; ++hq) {
