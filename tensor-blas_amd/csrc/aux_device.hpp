@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "masked_load.hpp"
 #include "tb_kernels.hpp"
 
 namespace tb {
@@ -82,25 +83,6 @@ __device__ __forceinline__ void rowexp_k_block(const T* __restrict__ X, long lon
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
     if (lane == 0 && e != kExpZero) atomicMax(out + r, e);
-}
-
-// x = *p if keep, else 0, converted to double, with the load itself unconditional (p
-// must be a valid address either way).  The mask goes through an empty (non-volatile)
-// asm so that the compiler cannot turn the masking back into a branch around the
-// load: a load behind a branch is waited for where the branch rejoins, which
-// serializes a row of loads.
-template <typename T>
-__device__ __forceinline__ double masked_load(const T* p, bool keep)
-{
-    unsigned m = keep ? 0xffffffffu : 0u;
-    asm("" : "+v"(m));
-    if constexpr (sizeof(T) == 4) {
-        return (double)__uint_as_float(*reinterpret_cast<const unsigned*>(p) & m);
-    } else {
-        const unsigned* w = reinterpret_cast<const unsigned*>(p);
-        const unsigned lo = w[0] & m, hi = w[1] & m;
-        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-    }
 }
 
 // Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.

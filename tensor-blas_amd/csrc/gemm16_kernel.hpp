@@ -497,30 +497,52 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
             const int q = b * FM + a;
+            // each group's reads of C (and of the column scales) are all issued before
+            // the first is used: behind per-element branches they went out one memory
+            // latency apart.  Groups of 8; one at a time with three accumulator sets live,
+            // whose registers leave no room for a group (the 3-plane batch kernels spill
+            // even at 4).
+            constexpr int G = kLo ? 1 : 8;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int jl = wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int j = j0 + jl;
-                float v = acc0[q][r];
-                if constexpr (kSplit) {
-                    float t = acc1[q][r];
-                    if constexpr (kLo) t = __builtin_fmaf(acc2[q][r], 1.0f / 2048.0f, t);
-                    v = __builtin_ldexpf(__builtin_fmaf(t, 1.0f / 2048.0f, v), -(scA + scB_s[jl]));
+            for (int hf = 0; hf < 16 / G; ++hf) {
+                bool in[G];
+                float cold[G];
+                double cs[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const int r = hf * G + u;
+                    const int j = j0 + wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    in[u] = i < M && j < N && (!lowerC || i >= j);
+                    cold[u] = (float)masked_load(Cb + (in[u] ? (long long)i + (long long)j * ldc : 0ll),
+                                                 in[u] && beta != 0.0f);
+                    cs[u] = P.cscale ? masked_load(P.cscale + (in[u] ? j : 0), in[u]) : 1.0;
                 }
-                if (i < M && j < N && (!lowerC || i >= j)) {
-                    float* c = Cb + (long long)i + (long long)j * ldc;
-                    float w;
-                    if (P.cscale) {
-                        const double cold = (beta == 0.0f) ? 0.0 : (double)*c;
-                        w = (float)(((double)alpha * (double)v + (double)beta * cold) * P.cscale[j]);
-                    } else {
-                        w = (beta == 0.0f) ? alpha * v : alpha * v + beta * *c;
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const int r = hf * G + u;
+                    const int jl = wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    const int j = j0 + jl;
+                    float v = acc0[q][r];
+                    if constexpr (kSplit) {
+                        float t = acc1[q][r];
+                        if constexpr (kLo) t = __builtin_fmaf(acc2[q][r], 1.0f / 2048.0f, t);
+                        v = __builtin_ldexpf(__builtin_fmaf(t, 1.0f / 2048.0f, v), -(scA + scB_s[jl]));
                     }
-                    *c = w;
-                    if (track) {
-                        const uint32_t u = __float_as_uint(w) & 0x7fffffffu;
-                        const int e = u == 0 ? kExpZeroDev : (int)(u >> 23) - 126;
-                        emax = e > emax ? e : emax;
+                    if (in[u]) {
+                        float* c = Cb + (long long)i + (long long)j * ldc;
+                        float w;
+                        if (P.cscale) {
+                            const double co = (beta == 0.0f) ? 0.0 : (double)cold[u];
+                            w = (float)(((double)alpha * (double)v + (double)beta * co) * cs[u]);
+                        } else {
+                            w = (beta == 0.0f) ? alpha * v : alpha * v + beta * cold[u];
+                        }
+                        *c = w;
+                        if (track) {
+                            const uint32_t bits = __float_as_uint(w) & 0x7fffffffu;
+                            const int e = bits == 0 ? kExpZeroDev : (int)(bits >> 23) - 126;
+                            emax = e > emax ? e : emax;
+                        }
                     }
                 }
             }

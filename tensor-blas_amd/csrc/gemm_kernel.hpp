@@ -21,6 +21,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "masked_load.hpp"
 #include "tb_common.hpp"
 
 namespace tb {
@@ -458,15 +459,32 @@ __device__ __forceinline__ void gemm_grouped_tile(const GemmLaunch& L, int g)
     const long long ldc = P.ldc;
     const bool track = (P.eOut != nullptr);
     int emax = kExpZeroDev;   // exponent bound of this lane's current row (FP32 tiers)
-    auto put = [&](int i, int j, Out v) {
-        if (i < M && j < N && (!lowerC || i >= j)) {
+    // C elements are written in groups: first every read of the group (C, and the
+    // column scales) is issued, unconditionally at clamped addresses, then the values
+    // are combined and stored; reads behind per-element branches went out one memory
+    // latency apart
+    struct CIn {
+        bool in;
+        Out cold;
+        double cs;
+    };
+    auto fetch = [&](int i, int j) {
+        CIn f;
+        f.in = i < M && j < N && (!lowerC || i >= j);
+        f.cold = (Out)masked_load(Cb + (f.in ? (long long)i + (long long)j * ldc : 0ll),
+                                  f.in && beta != Out(0));
+        f.cs = P.cscale ? masked_load(P.cscale + (f.in ? j : 0), f.in) : 1.0;
+        return f;
+    };
+    auto put = [&](int i, int j, Out v, const CIn& f) {
+        if (f.in) {
             Out* c = Cb + (long long)i + (long long)j * ldc;
             Out w;
             if (P.cscale) {
-                const double cold = (beta == Out(0)) ? 0.0 : (double)*c;
-                w = (Out)(((double)alpha * (double)v + (double)beta * cold) * P.cscale[j]);
+                const double cold = (beta == Out(0)) ? 0.0 : (double)f.cold;
+                w = (Out)(((double)alpha * (double)v + (double)beta * cold) * f.cs);
             } else {
-                w = (beta == Out(0)) ? alpha * v : alpha * v + beta * *c;
+                w = (beta == Out(0)) ? alpha * v : alpha * v + beta * f.cold;
             }
             *c = w;
             if constexpr (sizeof(Out) == 4) {
@@ -493,13 +511,17 @@ __device__ __forceinline__ void gemm_grouped_tile(const GemmLaunch& L, int g)
                 const int il = wm * WTM + a * 32 + (lane & 31);
                 int scA = 0;
                 if constexpr (TIER == kSplitF16) scA = scA_s[il];
+                CIn f[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    f[r] = fetch(i0 + il, j0 + wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3));
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int jl = wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                     float v = acc[q][r];
                     if constexpr (TIER == kSplitF16)
                         v = __builtin_ldexpf(v + accx[q][r] * (1.0f / 2048.0f), -(scA + scB_s[jl]));
-                    put(i0 + il, j0 + jl, v);
+                    put(i0 + il, j0 + jl, v, f[r]);
                 }
                 flush_row(il);
             }
@@ -510,20 +532,23 @@ __device__ __forceinline__ void gemm_grouped_tile(const GemmLaunch& L, int g)
             for (int a = 0; a < FM64; ++a) {
                 const int q = b * FM64 + a;
                 const int i = i0 + wm * WTM + a * 16 + (lane & 15);
+                // v_mfma_f64_16x16x4: accumulator r of lane l is row l/16 + 4r, column l%16
+                CIn f[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    // v_mfma_f64_16x16x4: accumulator r of lane l is row l/16 + 4r, column l%16
-                    const int j = j0 + wn * WTN + b * 16 + (lane >> 4) + 4 * r;
-                    put(i, j, acc64[q][r]);
-                }
+                for (int r = 0; r < 4; ++r) f[r] = fetch(i, j0 + wn * WTN + b * 16 + (lane >> 4) + 4 * r);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) put(i, j0 + wn * WTN + b * 16 + (lane >> 4) + 4 * r, acc64[q][r], f[r]);
             }
     } else {
         const int lr = lane & 15, lc = lane >> 4;
 #pragma unroll
         for (int a = 0; a < RM32; ++a) {
+            const int i = i0 + wm * WTM + lr + 16 * a;
+            CIn f[CN32];
 #pragma unroll
-            for (int b = 0; b < CN32; ++b)
-                put(i0 + wm * WTM + lr + 16 * a, j0 + wn * WTN + lc + 4 * b, acc32[b * RM32 + a]);
+            for (int b = 0; b < CN32; ++b) f[b] = fetch(i, j0 + wn * WTN + lc + 4 * b);
+#pragma unroll
+            for (int b = 0; b < CN32; ++b) put(i, j0 + wn * WTN + lc + 4 * b, acc32[b * RM32 + a], f[b]);
             flush_row(wm * WTM + lr + 16 * a);
         }
     }
