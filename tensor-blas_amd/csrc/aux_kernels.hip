@@ -306,10 +306,15 @@ xform_kernel(const T* __restrict__ src, long long lds, T* __restrict__ dst, long
     const int a0 = rr ? rows - bi - 32 : bi, b0 = rc ? cols - bj - 32 : bj;
     const int sr0 = tr ? b0 : a0, sc0 = tr ? a0 : b0;             // src block origin
     const int srows = tr ? cols : rows, scols = tr ? rows : cols;
-    for (int y = ty; y < 32; y += 8) {
-        const int si = sr0 + tx, sj = sc0 + y;
-        t[y][tx] = (si >= 0 && si < srows && sj >= 0 && sj < scols) ? src[si + (long long)sj * lds] : T(0);
+    T v[4];                                   // all four loads in flight (clamped, masked)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int si = sr0 + tx, sj = sc0 + ty + 8 * q;
+        const bool in = si >= 0 && si < srows && sj >= 0 && sj < scols;
+        v[q] = (T)masked_load(src + (in ? si + (long long)sj * lds : 0ll), in);
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[ty + 8 * q][tx] = v[q];
     __syncthreads();
     for (int y = ty; y < 32; y += 8) {
         const int i = bi + tx, j = bj + y;
