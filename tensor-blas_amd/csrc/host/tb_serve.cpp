@@ -80,7 +80,7 @@ double req_flops(const tb_request& r)
 // Ozaki): a covers launches and conversions, b the dependent chain of a recursion
 // (TRSM's n/nb leaves), c the rate of its products.  Fitted (non-negative least
 // squares on relative error) to isolated calls of every routine and tier at n = 128 ..
-// 8192 (k = n, m = n); median error 3-6 %, TRSM 12-16 %.
+// 8192 (k = n, m = n); median error 2-7 %, TRSM 11-18 %.
 // The FP32 and FP64 comparator tiers use a fixed cost and a flat rate.
 struct TimeModel {
     double a_us, b_us_per_n, c_tflops;
@@ -88,11 +88,11 @@ struct TimeModel {
 double est_us(const tb_request& r)
 {
     static const TimeModel model[5][3] = {
-        /* SYRK  */ {{30.4, 0.0038, 614.0}, {51.9, 0.0141, 270.0}, {67.6, 0.0297, 44.2}},
-        /* SYR2K */ {{40.8, 0.0061, 614.0}, {92.8, 0.0181, 251.0}, {103.2, 0.104, 46.5}},
-        /* TRMM  */ {{38.0, 0.0090, 425.0}, {67.2, 0.0128, 152.0}, {73.7, 0.0085, 23.5}},
-        /* TRSM  */ {{105.7, 0.452, 1e9}, {99.0, 0.527, 397.0}, {0.0, 0.984, 63.7}},
-        /* SYMM  */ {{37.6, 0.0083, 674.0}, {68.1, 0.0149, 270.0}, {77.8, 0.0074, 44.0}},
+        /* SYRK  */ {{26.0, 0.0055, 651.0}, {48.4, 0.0163, 304.0}, {60.2, 0.0305, 47.2}},
+        /* SYR2K */ {{29.9, 0.0095, 680.0}, {82.2, 0.0234, 279.0}, {88.2, 0.115, 50.4}},
+        /* TRMM  */ {{27.6, 0.0096, 409.0}, {61.0, 0.0115, 153.0}, {65.4, 0.0087, 24.7}},
+        /* TRSM  */ {{102.9, 0.381, 1e9}, {99.0, 0.452, 453.0}, {1.2, 0.973, 92.7}},
+        /* SYMM  */ {{28.1, 0.0090, 691.0}, {60.2, 0.0202, 297.0}, {65.4, 0.0163, 46.9}},
     };
     const double f = req_flops(r);
     if (r.tier == TB_TIER_FP32) return 50.0 + f / (70.0 * 1e6);
