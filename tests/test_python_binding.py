@@ -49,3 +49,67 @@ def test_layout_detection(TB):
     assert TB._layout(torch.zeros(8, 8)[::2, ::2]) is None
     a, ld = TB._as(t, "C")
     assert TB._layout(a) == ("C", 5) and ld == 5 and torch.equal(a, t)
+
+
+class _Recorder:
+    """stands in for tensorblas.Handle: records the C-ABI calls the binding makes"""
+
+    def __init__(self):
+        self.calls = []
+
+    def __getattr__(self, name):
+        def rec(*args, **kw):
+            self.calls.append((name, args, kw))
+            return 0
+        return rec
+
+
+@pytest.fixture
+def rec(TB, monkeypatch):
+    r = _Recorder()
+    monkeypatch.setattr(TB, "_handle", lambda t: r)
+    monkeypatch.setattr(TB, "_check", lambda tier, *ts: None)    # CPU tensors pass
+    return r
+
+
+def test_row_major_calls_pass_order_R_and_row_strides(TB, rec):
+    A = torch.zeros(6, 6)
+    B = torch.zeros(6, 9)
+    TB.trsm(A, B, side="L", uplo="U", trans="T", tier="split")
+    name, args, kw = rec.calls[-1]
+    assert name == "trsm_ex" and kw["order"] == "R"
+    assert args[1:5] == ("L", "U", "T", "N") and args[5:7] == (6, 9)
+    assert args[9] == 6 and args[11] == 9                         # lda, ldb: row strides
+
+
+def test_output_order_decides_and_inputs_are_copied(TB, rec):
+    A = torch.zeros(5, 5)                                         # row-major input
+    B = torch.zeros(7, 5).T                                       # column-major 5 x 7 output
+    TB.trsm(A, B, side="L", tier="split")
+    name, args, kw = rec.calls[-1]
+    assert kw["order"] == "C"
+    assert args[9] == 5 and args[11] == 5                         # A copied column-major
+
+
+def test_potrf_and_trtri_triangles(TB, rec):
+    # column-major memory of a row-major tensor is its transpose: a row-major upper
+    # triangle is the library's (column-major) lower one
+    A = torch.zeros(4, 4)
+    TB.potrf(A, upper=True, tier="split")
+    assert rec.calls[-1][0] == "potrf"
+    TB.trtri(A, upper=False, tier="split")
+    assert rec.calls[-1][0] == "trtri" and rec.calls[-1][1][1] == "U"
+    TB.trtri(A.T, upper=False, tier="split")                      # column-major view
+    assert rec.calls[-1][1][1] == "L"
+    n_before = len(rec.calls)
+    TB.potrf(A, upper=False, tier="split")                        # the other triangle:
+    assert rec.calls[n_before][0] == "potrf"                      # factors a column-major copy
+    assert rec.calls[n_before][1][3] == 4
+
+
+def test_syrk_shapes_follow_trans(TB, rec):
+    A = torch.zeros(3, 8)                                         # k x n for trans T
+    C = TB.syrk(A, trans="T", tier="split")
+    assert tuple(C.shape) == (8, 8)
+    name, args, kw = rec.calls[-1]
+    assert name == "syrk_ex" and args[3:5] == (8, 3) and kw["order"] == "R"
