@@ -203,20 +203,29 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
     const int tid = threadIdx.x;
     // the 16 elements of this thread: all loads issued before the first LDS store
     // (unconditional loads at a clamped address, then masked: loads behind per-element
-    // branches went out one memory latency apart)
+    // branches went out one memory latency apart).  A lower-stored operand's blocks
+    // wholly below (above) the diagonal read as mode N (as the mirror, mode T; TriL:
+    // zeros), with rows (k) contiguous across threads, so every read coalesces: with
+    // all loads in flight at once, strided reads would fetch each line many times.
+    int em = mode;
+    if (mode == kOpTriL || mode == kOpSymL) {
+        if (c0 + 63 - kshift <= r0) em = kOpN;
+        else if (c0 - kshift > r0 + 63) em = mode == kOpSymL ? kOpT : -1;
+    }
+    const bool kmaj = em == kOpT;
     float v[16];
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
         const int q = tid + it * 256;
-        const int rl = mode == kOpT ? q / 64 : q % 64, cl = mode == kOpT ? q % 64 : q / 64;
+        const int rl = kmaj ? q / 64 : q % 64, cl = kmaj ? q % 64 : q / 64;
         const int r = r0 + rl, k = c0 + cl - kshift;
-        bool keep = r < rows && k >= 0 && k < K;
+        bool keep = r < rows && k >= 0 && k < K && em >= 0;
         long long idx;
-        if (mode == kOpN) {
+        if (em == kOpN) {
             idx = (long long)r + (long long)k * ld;
-        } else if (mode == kOpT) {
+        } else if (em == kOpT) {
             idx = (long long)k + (long long)r * ld;
-        } else {                               // lower-stored: the other triangle mirrored
+        } else {                               // across the diagonal: the other triangle mirrored
             keep = keep && (r >= k || mode == kOpSymL);
             idx = r >= k ? (long long)r + (long long)k * ld : (long long)k + (long long)r * ld;
         }
@@ -225,7 +234,7 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
         const int q = tid + it * 256;
-        t[mode == kOpT ? q / 64 : q % 64][mode == kOpT ? q % 64 : q / 64] = v[it];
+        t[kmaj ? q / 64 : q % 64][kmaj ? q % 64 : q / 64] = v[it];
     }
     __syncthreads();
     const int rl = tid >> 2, cl0 = (tid & 3) * 16;

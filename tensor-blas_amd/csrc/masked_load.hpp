@@ -25,12 +25,5 @@ __device__ __forceinline__ double masked_load(const T* p, bool keep)
 }
 
 
-// the same for a 32-bit integer: *p if keep, else 0
-__device__ __forceinline__ int masked_load_i32(const int* p, bool keep)
-{
-    unsigned m = keep ? 0xffffffffu : 0u;
-    asm("" : "+v"(m));
-    return (int)(*reinterpret_cast<const unsigned*>(p) & m);
-}
 
 }  // namespace tb

@@ -44,32 +44,20 @@ ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int row
     const double* Xq = X + (long long)q * sx;
     const int r0 = blockIdx.y * kSliceTile, k0 = blockIdx.x * kSliceTile;
     const int tid = threadIdx.x;
-    // load: mode N / TriL / SymL-lower read rows contiguous, mode T reads k contiguous;
-    // all 16 loads of a thread are issued before the LDS stores (clamped, masked)
-    constexpr int kPer = kSliceTile * kSliceTile / 256;
-    double v[kPer];
-#pragma unroll
-    for (int it = 0; it < kPer; ++it) {
-        const int e = tid + it * 256;
-        const int rl = mode == kOpT ? e / kSliceTile : e % kSliceTile;
-        const int kl = mode == kOpT ? e % kSliceTile : e / kSliceTile;
+    // load: mode N / TriL / SymL-lower read rows contiguous, mode T reads k contiguous
+    for (int e = tid; e < kSliceTile * kSliceTile; e += 256) {
+        int rl, kl;
+        if (mode == kOpT) { kl = e % kSliceTile; rl = e / kSliceTile; }
+        else { rl = e % kSliceTile; kl = e / kSliceTile; }
         const int r = r0 + rl, k = k0 + kl;
-        bool keep = r < rows && k < K;
-        long long idx;
-        if (mode == kOpN) {
-            idx = (long long)r + (long long)k * ld;
-        } else if (mode == kOpT) {
-            idx = (long long)k + (long long)r * ld;
-        } else {                               // lower-stored: the other triangle mirrored
-            keep = keep && (r >= k || mode == kOpSymL);
-            idx = r >= k ? (long long)r + (long long)k * ld : (long long)k + (long long)r * ld;
+        double v = 0.0;
+        if (r < rows && k < K) {
+            if (mode == kOpN) v = Xq[(long long)r + (long long)k * ld];
+            else if (mode == kOpT) v = Xq[(long long)k + (long long)r * ld];
+            else if (r >= k) v = Xq[(long long)r + (long long)k * ld];
+            else if (mode == kOpSymL) v = Xq[(long long)k + (long long)r * ld];
         }
-        v[it] = masked_load(Xq + (keep ? idx : 0ll), keep);
-    }
-#pragma unroll
-    for (int it = 0; it < kPer; ++it) {
-        const int e = tid + it * 256;
-        t[mode == kOpT ? e / kSliceTile : e % kSliceTile][mode == kOpT ? e % kSliceTile : e / kSliceTile] = v[it];
+        t[rl][kl] = v;
     }
     __syncthreads();
     // each thread: one row, 16 consecutive k; rows past `rows` (up to the next multiple
@@ -318,28 +306,15 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
         if (i < P.m) {
             const int ea = P.ea[i];
             const int sea = ea < -5000 ? 0 : ea;
-            // every read of the lane's 16 elements (column exponents, C) is issued before
-            // the first is used (clamped, masked): behind per-element branches they went
-            // out one memory latency apart
-            bool in[16];
-            int ebv[16];
-            double cold[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int j = j0 + wn * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                in[r] = j < P.n && !(P.lowerC && i < j);
-                ebv[r] = masked_load_i32(P.eb + (in[r] ? j : 0), in[r]);
-                cold[r] = masked_load(P.C + (in[r] ? (long long)i + (long long)j * P.ldc : 0ll),
-                                      in[r] && P.beta != 0.0);
-            }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                if (!in[r]) continue;
-                const int j = j0 + wn * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int seb = ebv[r] < -5000 ? 0 : ebv[r];
+                if (j >= P.n || (P.lowerC && i < j)) continue;
+                const int eb = P.eb[j];
+                const int seb = eb < -5000 ? 0 : eb;
                 const double v = __builtin_ldexp(facc[r], sea + seb);
                 double* c = P.C + (long long)i + (long long)j * P.ldc;
-                *c = (P.beta == 0.0) ? P.alpha * v : __builtin_fma(P.alpha, v, P.beta * cold[r]);
+                *c = (P.beta == 0.0) ? P.alpha * v : __builtin_fma(P.alpha, v, P.beta * *c);
             }
         }
     }
