@@ -73,6 +73,29 @@ def test_invalid_arguments_rejected_without_device():
     assert L.tb_set_ozaki_slices(null, 7) == 1
     assert L.tb_status_string(1) == b"invalid argument"
     assert L.tb_version() >= 100
+    # the option-set, factorization and inverse entry points check the handle first
+    c = ctypes.c_char
+    assert L.tb_trsm_ex(null, 1, c(b"C"), c(b"L"), c(b"L"), c(b"N"), c(b"N"), 8, 8, 1.0, 1, 8, 1, 8, 256) == 1
+    assert L.tb_syrk_ex(null, 1, c(b"R"), c(b"U"), c(b"T"), 8, 8, 1.0, 1, 8, 0.0, 1, 8, 256) == 1
+    info = ctypes.c_long(0)
+    assert L.tb_potrf(null, 1, 8, 1, 8, 256, ctypes.byref(info)) == 1
+    assert L.tb_trtri(null, 1, c(b"L"), c(b"N"), 8, 1, 8, 256, ctypes.byref(info)) == 1
+
+
+@pytest.mark.parametrize("std,lang,compiler", [("c99", "c", "gcc"), ("c++17", "c++", "g++")])
+def test_header_compiles_standalone(tmp_path, std, lang, compiler):
+    """include/TensorBLAS.h is the boundary a C, cgo or JNI caller compiles against:
+    it must build on its own, warning-free, as C99 and as C++."""
+    import os
+    import shutil
+    if shutil.which(compiler) is None:
+        pytest.skip(f"{compiler} not installed")
+    src = tmp_path / ("h." + ("c" if lang == "c" else "cpp"))
+    src.write_text('#include "TensorBLAS.h"\nint main(void) { return TB_SUCCESS; }\n')
+    r = subprocess.run([compiler, f"-std={std}", "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.dirname(HEADER), "-c", str(src), "-o", str(tmp_path / "h.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
 
 
 @pytest.mark.parametrize("n", [0, 1, 2048, 8191, 8192, 16384, 20000, 32768, 40960, 188416])
