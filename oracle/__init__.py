@@ -53,6 +53,9 @@ def lib():
                                      _dbl, _p, _i64, _i64, ctypes.c_int]
         L.orc_errors.argtypes = [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, ctypes.c_int, _p]
         L.orc_dpotrf_lower.argtypes = [_i64, _p, _i64]
+        L.orc_rowdot_xp.argtypes = [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _i64, _i64,
+                                    _p, _p]
+        L.orc_rowdot_xp.restype = None
         L.orc_dpotrf_lower.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -231,3 +234,26 @@ def dpotrf_lower(A):
     info = lib().orc_dpotrf_lower(n, _ptr(L), max(1, n))
     return L, int(info)
 
+
+
+# ------------------------------------------------- extended-precision columns
+def rowdot_xp(X, Y, cols, X2=None, Y2=None):
+    """out[:, c] = X @ Y[cols[c]] (+ X2 @ Y2[cols[c]]) with products and sums in long
+    double (64-bit significand), rounded once to FP64.  X, Y, X2, Y2: 2-D float64
+    arrays whose rows are the dot-product vectors (C order).  SYR2K's column j is
+    rowdot_xp(A, B, [j], B, A) (n x k operands); column j of A @ B is
+    rowdot_xp(A, B.T, [j])."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    n, k = X.shape
+    assert Y.shape[1] == k
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    assert cols.min(initial=0) >= 0 and cols.max(initial=0) < Y.shape[0]
+    if X2 is not None:
+        X2 = np.ascontiguousarray(X2, dtype=np.float64)
+        Y2 = np.ascontiguousarray(Y2, dtype=np.float64)
+        assert X2.shape == X.shape and Y2.shape[1] == k and Y2.shape[0] == Y.shape[0]
+    out = np.zeros((len(cols), n))
+    lib().orc_rowdot_xp(n, k, _ptr(X), k, _ptr(Y), k, _ptr(X2) if X2 is not None else None, k,
+                        _ptr(Y2) if Y2 is not None else None, k, len(cols), _ptr(cols), _ptr(out))
+    return out.T.copy()

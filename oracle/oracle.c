@@ -26,10 +26,15 @@
  *      SYRK and TRSM.  No reference counterpart (the reference stops at the BLAS-3
  *      routines); LAPACK dpotrf semantics, pinned by tests against numpy.linalg.cholesky
  *      and by the reconstruction L L^T = A.
+ *   8. sampled columns of the FP64 products in extended precision (x87 long double,
+ *      64-bit significand): the reference for the Ozaki tier's 1e-14 target at full size
+ *      (BLAS/tc_ozimmu_syr2k.cu:3-31 computes them in FP64).  Pinned by exact rational
+ *      arithmetic on small inputs.
  *
  * All matrices are column-major, leading dimensions in elements.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -450,4 +455,47 @@ int orc_dpotrf_lower(int64_t n, double *A, int64_t lda)
         }
     }
     return 0;
+}
+
+/* ------------------------------------------- 8. sampled columns, extended precision */
+
+/* out[c*n + i] = sum_p X[i,p] Y[cols[c],p] (+ X2[i,p] Y2[cols[c],p] when X2 != NULL)
+ * for i < n and c < ncols, with the rows of X, Y, X2, Y2 contiguous (row r of X at
+ * X + r*ldx, k elements: row-major storage).  Products and sums are formed in long
+ * double (64-bit significand on x86-64), then rounded once to double.
+ *   SYR2K column j (lower part, i >= j): X = A, Y = B, X2 = B, Y2 = A, cols = {j}.
+ *   SYMM / GEMM column j of A*B: X = rows of A, Y = rows of B^T.
+ * Rows are shared among OpenMP threads; the sampled rows of Y, Y2 stay in cache. */
+void orc_rowdot_xp(int64_t n, int64_t k, const double *X, int64_t ldx, const double *Y,
+                   int64_t ldy, const double *X2, int64_t ldx2, const double *Y2, int64_t ldy2,
+                   int64_t ncols, const int64_t *cols, double *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const double *x = X + i * ldx;
+        const double *x2 = X2 ? X2 + i * ldx2 : NULL;
+        for (int64_t c = 0; c < ncols; c++) {
+            const double *y = Y + cols[c] * ldy;
+            long double s0 = 0.0L, s1 = 0.0L, s2 = 0.0L, s3 = 0.0L;
+            int64_t p = 0;
+            for (; p + 4 <= k; p += 4) {
+                s0 += (long double)x[p] * y[p];
+                s1 += (long double)x[p + 1] * y[p + 1];
+                s2 += (long double)x[p + 2] * y[p + 2];
+                s3 += (long double)x[p + 3] * y[p + 3];
+            }
+            for (; p < k; p++) s0 += (long double)x[p] * y[p];
+            if (x2) {
+                const double *y2 = Y2 + cols[c] * ldy2;
+                for (p = 0; p + 4 <= k; p += 4) {
+                    s0 += (long double)x2[p] * y2[p];
+                    s1 += (long double)x2[p + 1] * y2[p + 1];
+                    s2 += (long double)x2[p + 2] * y2[p + 2];
+                    s3 += (long double)x2[p + 3] * y2[p + 3];
+                }
+                for (; p < k; p++) s0 += (long double)x2[p] * y2[p];
+            }
+            out[c * n + i] = (double)((s0 + s1) + (s2 + s3));
+        }
+    }
 }

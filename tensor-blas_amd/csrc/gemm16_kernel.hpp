@@ -56,13 +56,14 @@ __device__ __forceinline__ int swz(int row, int unit) { return unit ^ ((row >> 2
 // vmcnt(0) for it (it cannot tell the ring slots apart); the kernel's own
 // TB_WAIT_VM_BARRIER accounting orders it.  The instruction has no result register;
 // its address registers are consumed at issue.  lds_base is wave-uniform; it goes
-// through readfirstlane so that it lands in the SGPR m0 is loaded from.
+// through readfirstlane and is bound to m0 by the "{m0}" operand constraint, so the
+// compiler itself writes m0 (no clobber of a reserved register).
 __device__ __forceinline__ void lds_dma16(const void* g, unsigned lds_base)
 {
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+    asm volatile("global_load_lds_dwordx4 %1, off"
                  :
-                 : "s"(__builtin_amdgcn_readfirstlane(lds_base)), "v"(g)
-                 : "memory", "m0");
+                 : "{m0}"(__builtin_amdgcn_readfirstlane(lds_base)), "v"(g)
+                 : "memory");
 }
 
 // One operand panel of R rows x kBK with NPL planes.

@@ -441,17 +441,21 @@ int autotune(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long 
     std::vector<void*> scratch;
     int st = TB_SUCCESS;
     for (int i = 0; i < count && st == TB_SUCCESS; ++i) {
-        const long cols = E[i].op == TB_OP_SYRK ? E[i].n : E[i].n;
-        const size_t bytes = sizeof(float) * (size_t)E[i].ldc * (size_t)std::max(1l, cols);
-        if (!E[i].C || cols == 0) continue;
+        // the output is rows x cols with leading dimension ldc: only (cols-1)*ldc + rows
+        // elements of it are the caller's, so it is copied as a 2D region
+        const long rows = E[i].op == TB_OP_SYRK ? E[i].n : E[i].m, cols = E[i].n;
+        if (!E[i].C || rows <= 0 || cols <= 0) continue;
         void* p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
+        if (hipMalloc(&p, sizeof(float) * (size_t)E[i].ldc * (size_t)cols) != hipSuccess) {
             (void)hipGetLastError();
             st = TB_ERR_NOMEM;
             break;
         }
         scratch.push_back(p);
-        if (hipMemcpy(p, E[i].C, bytes, hipMemcpyDeviceToDevice) != hipSuccess) st = TB_ERR_DEVICE;
+        const size_t pitch = sizeof(float) * (size_t)E[i].ldc;
+        if (hipMemcpy2D(p, pitch, E[i].C, pitch, sizeof(float) * (size_t)rows, (size_t)cols,
+                        hipMemcpyDeviceToDevice) != hipSuccess)
+            st = TB_ERR_DEVICE;
         T[i].C = static_cast<float*>(p);
     }
     tb_batch_s* t = new (std::nothrow) tb_batch_s();

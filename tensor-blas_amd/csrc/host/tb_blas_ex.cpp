@@ -218,6 +218,8 @@ int symm_ex(tb_handle_s* h, int tier, char order, char side, char uplo, long m, 
     const long na = sd == 'L' ? m : n;
     if (lda < std::max(1l, na) || ldb < std::max(1l, m) || ldc < std::max(1l, m)) return TB_ERR_INVALID;
     if (m == 0 || n == 0) return TB_SUCCESS;
+    // null operands are rejected before any operand copy launches
+    if (!A || !B || !C) return TB_ERR_INVALID;
     if (sd == 'L' && ul == 'L') return symm(h, tier, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
     const bool f64 = f64_of(tier);
     const size_t es = f64 ? 8 : 4;
@@ -269,6 +271,7 @@ int syrk_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n,
         return TB_ERR_INVALID;
     if (ldc < std::max(1l, n) || lda < std::max(1l, tr == 'N' ? n : k)) return TB_ERR_INVALID;
     if (n == 0) return TB_SUCCESS;
+    if (!C || (k > 0 && !A)) return TB_ERR_INVALID;
     const bool f64 = f64_of(tier);
     ExWs w{h, f64 ? 8u : 4u};
     const size_t oC = ul == 'U' ? w.add(n, n) : 0;
@@ -289,6 +292,7 @@ int syr2k_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n
     const long ar = tr == 'N' ? n : k;
     if (ldc < std::max(1l, n) || lda < std::max(1l, ar) || ldb < std::max(1l, ar)) return TB_ERR_INVALID;
     if (n == 0) return TB_SUCCESS;
+    if (!C || (k > 0 && (!A || !B))) return TB_ERR_INVALID;
     const bool f64 = f64_of(tier);
     ExWs w{h, f64 ? 8u : 4u};
     const size_t oA = tr == 'T' ? w.add(n, k) : 0;

@@ -110,10 +110,10 @@ constexpr int oz_stages()
 // direct-to-LDS load of 16 bytes per lane (see gemm16_kernel.hpp lds_dma16)
 __device__ __forceinline__ void oz_dma16(const void* g, unsigned lds_base)
 {
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+    asm volatile("global_load_lds_dwordx4 %1, off"
                  :
-                 : "s"(__builtin_amdgcn_readfirstlane(lds_base)), "v"(g)
-                 : "memory", "m0");
+                 : "{m0}"(__builtin_amdgcn_readfirstlane(lds_base)), "v"(g)
+                 : "memory");
 }
 // "s_waitcnt vmcnt(n)" (this wave's direct-to-LDS loads but the newest n are done),
 // lgkmcnt(0), then a workgroup barrier

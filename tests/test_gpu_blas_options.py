@@ -239,3 +239,32 @@ def test_options_arguments(h):
         with pytest.raises(tensorblas.TensorBLASError):
             f()
     h.syrk_ex("fp64", "U", "N", 0, 4, 1.0, p, 1, 0.0, p, 1)                   # empty: no-op
+
+
+@pytest.mark.parametrize("tier", ["split", "fp64"])
+def test_null_operands_rejected_before_any_copy(h, tier):
+    """Option sets that copy operands (symm side R or uplo U, syrk uplo U, syr2k trans T)
+    reject a null operand with TB_ERR_INVALID before the copy is launched; the caller's
+    other buffers are untouched."""
+    import tensorblas
+    L = tensorblas.lib()
+    c = tensorblas._c
+    t = tensorblas.tier_id(tier)
+    dt = torch.float64 if tier == "fp64" else torch.float32
+    d = torch.full((64,), 7.0, dtype=dt, device="cuda")
+    p = d.data_ptr()
+    hp = h.ptr
+    calls = [
+        lambda: L.tb_symm_ex(hp, t, c("C"), c("R"), c("L"), 4, 4, 1.0, None, 4, p, 4, 0.0, p, 4),
+        lambda: L.tb_symm_ex(hp, t, c("C"), c("L"), c("U"), 4, 4, 1.0, p, 4, None, 4, 0.0, p, 4),
+        lambda: L.tb_symm_ex(hp, t, c("C"), c("R"), c("U"), 4, 4, 1.0, p, 4, p, 4, 0.0, None, 4),
+        lambda: L.tb_syrk_ex(hp, t, c("C"), c("U"), c("N"), 4, 4, 1.0, p, 4, 0.0, None, 4, 256),
+        lambda: L.tb_syrk_ex(hp, t, c("C"), c("U"), c("T"), 4, 4, 1.0, None, 4, 0.0, p, 4, 256),
+        lambda: L.tb_syr2k_ex(hp, t, c("C"), c("L"), c("T"), 4, 4, 1.0, None, 4, p, 4, 0.0, p, 4, 256),
+        lambda: L.tb_syr2k_ex(hp, t, c("C"), c("U"), c("T"), 4, 4, 1.0, p, 4, None, 4, 0.0, p, 4, 256),
+        lambda: L.tb_syr2k_ex(hp, t, c("C"), c("U"), c("N"), 4, 4, 1.0, p, 4, p, 4, 0.0, None, 4, 256),
+    ]
+    for f in calls:
+        assert f() == 1
+    h.synchronize()
+    assert bool((d == 7.0).all())

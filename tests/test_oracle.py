@@ -402,3 +402,45 @@ def test_ozaki_gemm_all_groups_is_exact_product_of_slices():
             assert abs(Fraction(full[i, j]) - exact) <= scale * Fraction(1, 2 ** 50)
             assert abs(plus[i, j] - float(exact)) <= abs(tri[i, j] - float(exact)) + float(scale) * 2 ** -50
 
+
+
+# ------------------------------------------- extended-precision sampled columns
+def _spread(g, shape, phi):
+    return g.uniform(-0.5, 0.5, shape) * np.exp(phi * g.standard_normal(shape))
+
+
+@pytest.mark.parametrize("phi", [0.5, 2.0])
+def test_rowdot_xp_syr2k_columns_exact(phi):
+    """SYR2K columns against exact rational arithmetic: after one rounding to FP64 the
+    long-double sum is within 2^-53 |exact| + k 2^-62 sum|products| (products and
+    sums carry a 64-bit significand).  A dropped second product, a swapped operand or a
+    wrong column index is off by O(|C|)."""
+    g = rng(int(phi * 10) + 3)
+    n, k = 9, 37
+    A, B = _spread(g, (n, k), phi), _spread(g, (n, k), phi)
+    cols = [0, 4, 8, 2]
+    out = oracle.rowdot_xp(A, B, cols, B, A)
+    for c, j in enumerate(cols):
+        for i in range(n):
+            terms = [Fraction(float(A[i, p])) * Fraction(float(B[j, p])) +
+                     Fraction(float(B[i, p])) * Fraction(float(A[j, p])) for p in range(k)]
+            exact = sum(terms)
+            mag = sum(abs(Fraction(float(A[i, p])) * Fraction(float(B[j, p]))) +
+                      abs(Fraction(float(B[i, p])) * Fraction(float(A[j, p]))) for p in range(k))
+            bound = abs(exact) * Fraction(1, 2 ** 53) + mag * k * Fraction(1, 2 ** 62)
+            assert abs(Fraction(float(out[i, c])) - exact) <= bound, (i, j)
+
+
+def test_rowdot_xp_gemm_column_and_fp64_agreement():
+    """Column j of A @ B (X = A, Y = B^T) agrees with FP64 OpenBLAS to FP64 rounding,
+    and the single-product form has no second term."""
+    g = rng(5)
+    m, k, n = 300, 257, 40
+    A, B = g.uniform(-1, 1, (m, k)), g.uniform(-1, 1, (k, n))
+    cols = [0, 17, 39]
+    out = oracle.rowdot_xp(A, B.T, cols)
+    R = (A @ B)[:, cols]
+    assert np.max(np.abs(out - R)) <= 1e-13 * np.max(np.abs(R))
+    # exactly representable case: small integers give the exact product
+    Ai, Bi = g.integers(-50, 50, (m, k)).astype(np.float64), g.integers(-50, 50, (k, n)).astype(np.float64)
+    assert np.array_equal(oracle.rowdot_xp(Ai, Bi.T, cols), (Ai @ Bi)[:, cols])
