@@ -67,7 +67,8 @@ enum {
     TB_SUCCESS = 0,
     TB_ERR_INVALID = 1,   /* bad dimension, leading dimension, nb, tier or null pointer */
     TB_ERR_DEVICE = 2,    /* a HIP call failed (launch, allocation, copy) */
-    TB_ERR_NOMEM = 3      /* device workspace could not be allocated */
+    TB_ERR_NOMEM = 3,     /* device workspace could not be allocated */
+    TB_ERR_SCRATCH = 4    /* the problem needs more than the no-allocation scratch holds */
 };
 
 /* ------------------------------------------------------------- handle */
@@ -125,6 +126,20 @@ TB_API int tb_version(void);
 /* replaces tc_syrk / tc_cumpsgemm_syrk / tc_ozimmu_syrk (include/TensorBLAS.h:139-143) */
 TB_API int tb_syrk(tb_handle_t h, int tier, long n, long k, double alpha, const void* A, long lda,
             double beta, void* C, long ldc, long nb);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * tb_syrk for PLAIN, SPLIT, FP32 and FP64 that allocates nothing.  The FP16 tiers'
+ * operand planes and row exponents go into one fixed device array of
+ * TB_NOALLOC_SCRATCH_BYTES (64 MiB) per GPU, held from library load until the process
+ * ends, instead of the handle's workspace (which it never touches).  A split-tier
+ * problem needs about 4 n round_up(k, 64) + 4 n bytes of it (plain: half of the planes);
+ * larger problems return TB_ERR_SCRATCH before launching anything.  OZAKI returns
+ * TB_ERR_INVALID.  Every call on one GPU shares that GPU's array and leaves its contents
+ * behind, so calls that overlap on one GPU (two streams or threads) race.  No profiling
+ * records are taken.  Results equal tb_syrk's bit for bit. */
+#define TB_NOALLOC_SCRATCH_BYTES (64L << 20)
+TB_API int tb_syrk_noalloc(tb_handle_t h, int tier, long n, long k, double alpha, const void* A,
+                           long lda, double beta, void* C, long ldc, long nb);
 /* replaces tc_syr2k / tc_cumpsgemm_syr2k / tc_ozimmu_syr2k (include/TensorBLAS.h:159-163) */
 TB_API int tb_syr2k(tb_handle_t h, int tier, long n, long k, double alpha, const void* A, long lda,
              const void* B, long ldb, double beta, void* C, long ldc, long nb);

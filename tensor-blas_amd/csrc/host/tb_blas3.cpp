@@ -280,8 +280,11 @@ struct RowsOf {
     int mode;
 };
 
+// fixed_ws: when set, the planes and exponents go there (at most fixed_bytes) instead of
+// the handle's workspace, and nothing is allocated (TB_ERR_SCRATCH when they do not fit)
 int fp16_product(tb_handle_s* h, int tier, long m, long n, long k, double alpha, RowsOf a,
-                 RowsOf b, bool same_ab, bool pairs, double beta, float* C, long ldc, int modeC)
+                 RowsOf b, bool same_ab, bool pairs, double beta, float* C, long ldc, int modeC,
+                 char* fixed_ws = nullptr, size_t fixed_bytes = 0)
 {
     const int npl = planes_of(h, tier);
     const bool split = tier == kSplitF16;
@@ -292,7 +295,12 @@ int fp16_product(tb_handle_s* h, int tier, long m, long n, long k, double alpha,
     const size_t oPA = ar.reserve(planes_bytes(npl, m, k));
     const size_t oPB = sep_b ? ar.reserve(planes_bytes(npl, n, k)) : 0;
     char* ws;
-    TB_TRY(workspace_get(h, ar.total(), &ws));
+    if (fixed_ws) {
+        if (ar.total() > fixed_bytes) return TB_ERR_SCRATCH;
+        ws = fixed_ws;
+    } else {
+        TB_TRY(workspace_get(h, ar.total(), &ws));
+    }
     int* eA = reinterpret_cast<int*>(ws + oEA);
     int* eB = reinterpret_cast<int*>(ws + oEB);
     if (split && k > 0) {
@@ -373,6 +381,37 @@ int syrk(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A, 
     GemmQueue q(h, tier, kShapeSquare);
     TB_TRY(q.add(make_prob(n, n, k, A, lda, mA, A, lda, mB, C, ldc, alpha, beta, kCLower)));
     return q.flush();
+}
+
+// Same as syrk with trans_a = false, except that the FP16 tiers' operand planes and row
+// exponents live in the handle's fixed no-allocation scratch (kNoAllocScratchBytes per
+// device, shared by every call on that device) instead of the handle's workspace.  The
+// FP32 and FP64 tiers need no scratch; the Ozaki tier's slice workspace is not offered
+// here.  Profiling events are not recorded (creating them allocates).
+int syrk_noalloc(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A,
+                 long lda, double beta, void* C, long ldc, long nb)
+{
+    if (!h || bad_tier(tier) || tier == kOzakiI8 || !dims_ok(n) || !dims_ok(k) || nb <= 0)
+        return TB_ERR_INVALID;
+    if (lda < std::max(1l, n) || ldc < std::max(1l, n)) return TB_ERR_INVALID;
+    if (n == 0) return TB_SUCCESS;
+    if (!C || (k > 0 && !A)) return TB_ERR_INVALID;
+    static_assert(kNoAllocScratchBytes == (size_t)TB_NOALLOC_SCRATCH_BYTES, "scratch size");
+    if (!h->scratch) return TB_ERR_DEVICE;
+    const bool prof = h->prof;
+    h->prof = false;
+    int st;
+    if (fp16_tier(tier)) {
+        st = fp16_product(h, tier, n, n, k, alpha, {(const float*)A, lda, kOpN},
+                          {(const float*)A, lda, kOpN}, true, false, beta, (float*)C, ldc, kCLower,
+                          static_cast<char*>(h->scratch), kNoAllocScratchBytes);
+    } else {
+        GemmQueue q(h, tier, kShapeSquare);
+        st = q.add(make_prob(n, n, k, A, lda, kOpN, A, lda, kOpT, C, ldc, alpha, beta, kCLower));
+        if (st == TB_SUCCESS) st = q.flush();
+    }
+    h->prof = prof;
+    return st;
 }
 
 // ------------------------------------------------------------------- SYR2K

@@ -19,6 +19,12 @@ int tb_create(tb_handle_t* handle)
         delete h;
         return TB_ERR_DEVICE;
     }
+    // set-up of the no-allocation scratch: its device address on this (current) device
+    if (noalloc_scratch(h->device, &h->scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        delete h;
+        return TB_ERR_DEVICE;
+    }
     *handle = h;
     return TB_SUCCESS;
 }
@@ -125,6 +131,7 @@ const char* tb_status_string(int status)
     case TB_ERR_INVALID: return "invalid argument";
     case TB_ERR_DEVICE: return "device error";
     case TB_ERR_NOMEM: return "out of device memory";
+    case TB_ERR_SCRATCH: return "problem exceeds the no-allocation scratch";
     default: return "unknown status";
     }
 }
@@ -135,6 +142,12 @@ int tb_syrk(tb_handle_t h, int tier, long n, long k, double alpha, const void* A
             double beta, void* C, long ldc, long nb)
 {
     return syrk(h, tier, n, k, alpha, A, lda, false, beta, C, ldc, nb);
+}
+
+int tb_syrk_noalloc(tb_handle_t h, int tier, long n, long k, double alpha, const void* A, long lda,
+                    double beta, void* C, long ldc, long nb)
+{
+    return syrk_noalloc(h, tier, n, k, alpha, A, lda, beta, C, ldc, nb);
 }
 
 int tb_syr2k(tb_handle_t h, int tier, long n, long k, double alpha, const void* A, long lda,

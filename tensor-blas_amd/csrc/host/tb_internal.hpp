@@ -34,6 +34,7 @@ struct tb_handle_s {
     void* ws[kWsSlots] = {};
     size_t ws_bytes[kWsSlots] = {};
     int* dev_info = nullptr;   // POTRF / TRTRI status word (device), allocated on first use
+    void* scratch = nullptr;   // the fixed no-allocation scratch of this handle's device
 };
 
 namespace tb {
@@ -135,6 +136,10 @@ int syr2k_ex(tb_handle_s* h, int tier, char order, char uplo, char trans, long n
              long lda, const void* B, long ldb, double beta, void* C, long ldc, long nb);
 int trsm(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B, long ldb,
          long nb);
+// syrk (trans_a = false) with its operand planes and exponents in the handle's fixed
+// no-allocation scratch instead of the handle's workspace: no device allocation
+int syrk_noalloc(tb_handle_s* h, int tier, long n, long k, double alpha, const void* A,
+                 long lda, double beta, void* C, long ldc, long nb);
 int trsm_refined(tb_handle_s* h, int tier, long m, long n, const void* A, long lda, void* B,
                  long ldb, long nb);
 // schedule of TRSM (leaves and updates in execution order), for inspection

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "tb_common.hpp"
@@ -151,6 +152,14 @@ hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);
 constexpr int kPotrfLeaf = 128;
 hipError_t potrf_leaf(bool f64, void* A, long long lda, int s, long long off, int* info,
                       hipStream_t st);
+
+// ------------------------------------------------------ no-allocation scratch
+// Bytes of the fixed device scratch behind tb_syrk_noalloc (one array per GPU;
+// TB_NOALLOC_SCRATCH_BYTES of the public header).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// Device address of that array on `device`, which must be the current device: read
+// with hipGetSymbolAddress on the first call for the device, then cached.
+hipError_t noalloc_scratch(int device, void** out);
 
 // ---------------------------------------------------------------------- batches
 struct RowexpJob;

@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(_HERE, "..", "build", "libTensorBLAS.so")
 
 TIERS = {"plain": 0, "split": 1, "ozaki": 2, "fp32": 3, "fp64": 4}
 F64_TIERS = {"ozaki", "fp64"}
-STATUS = {0: "success", 1: "invalid argument", 2: "device error", 3: "out of device memory"}
+STATUS = {0: "success", 1: "invalid argument", 2: "device error", 3: "out of device memory",
+          4: "problem exceeds the no-allocation scratch"}
 
 
 class TensorBLASError(RuntimeError):
@@ -46,6 +47,7 @@ def lib():
             "tb_set_split_planes": [_P, _I],
             "tb_set_max_cus": [_P, _I],
             "tb_syrk": [_P, _I, _L, _L, _D, _P, _L, _D, _P, _L, _L],
+            "tb_syrk_noalloc": [_P, _I, _L, _L, _D, _P, _L, _D, _P, _L, _L],
             "tb_syr2k": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _L],
             "tb_trmm": [_P, _I, _L, _L, _D, _P, _L, _P, _L, _P, _L, _L],
             "tb_trsm": [_P, _I, _L, _L, _P, _L, _P, _L, _L],
@@ -173,6 +175,11 @@ class Handle:
     # ------------------------------------------------------------ routines
     def syrk(self, tier, n, k, alpha, A, lda, beta, C, ldc, nb=256):
         _check("tb_syrk", lib().tb_syrk(self._h, tier_id(tier), n, k, alpha, A, lda, beta, C, ldc, nb))
+
+    def syrk_noalloc(self, tier, n, k, alpha, A, lda, beta, C, ldc, nb=256):
+        """tb_syrk without device allocation (fixed per-GPU scratch; see the header)"""
+        _check("tb_syrk_noalloc", lib().tb_syrk_noalloc(self._h, tier_id(tier), n, k, alpha, A, lda,
+                                                        beta, C, ldc, nb))
 
     def syr2k(self, tier, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=256):
         _check("tb_syr2k", lib().tb_syr2k(self._h, tier_id(tier), n, k, alpha, A, lda, B, ldb, beta,
