@@ -3,18 +3,26 @@
 Default workload (the headline): recursive TRSM X*L^T = B at n = m = 16384,
 nb = 256.  The FP32-accurate split tier (FP16 hi/lo split on MFMA) is timed against
 the plain FP32 vector-FMA kernel of the same recursion on the same GPU.  The line
-also carries the plain FP16 tier, the max-relative and normwise error of each tier
-against an FP64 host solve of a sample of rows, the dominant kernel's roofline and
-host CPU baselines on a bounded sample of the same problem.
+also carries the plain FP16 tier, the split tier with FP64-residual refinement, the
+max-relative and normwise error of each tier against an FP64 host solve of a sample of
+rows, the dominant kernel's roofline and a host BLAS baseline on a bounded sample.
 
 One step = restore B from a resident copy (a device copy, 2 GiB of HBM traffic) +
 one tb_trsm call.  Inputs are resident in HBM before the timed region.
 
+After the headline the same process measures the benchmark's other configurations and
+carries each, with its own errors, roofline and host baseline, under "configs"
+(--no-configs skips them): SYRK n = k = 2048 (plain / split / FP32-FMA); split SYRK
+against FP32-FMA at n = k = 16384; Ozaki SYR2K 32768 x 8192 at phi 0.5 and 2 and Ozaki
+SYMM 32768 x 8192 at phi 0.5 against native FP64 MFMA; the 2000-instance batch plan.
+Then the request stream (--no-serve skips it) under "served".
+
 Other workloads (--workload):
-  syrk   plain SYRK C = A A^T, lower, n = k = 2048, nb = 256 (split and FP32-FMA
-         tiers beside it), error vs FP64 dsyrk, host OpenBLAS ssyrk baseline
-  ozaki  FP64 SYR2K by Ozaki INT8 slicing, n = 32768, k = 8192, against the native
-         FP64-MFMA tier; error vs an extended-precision host product on sampled columns
+  syrk   SYRK C = A A^T, lower, n = k = 2048 (--n, --k), nb = 256, plain / split /
+         FP32-FMA tiers, error vs FP64 BLAS, host OpenBLAS ssyrk baseline
+  ozaki  FP64 SYR2K (--op syr2k) or SYMM (--op symm) by Ozaki INT8 slicing, n = 32768,
+         k = 8192, against the native FP64-MFMA tier; error vs FP64 BLAS products of
+         64 sampled output columns
   batched  2000 independent SYRK / TRSM instances (n log-uniform in [64, 2048]) as
          one batch plan (persistent kernel, one HIP graph), against the same instances
          as single calls in the same tier and in the FP32-FMA tier
@@ -167,17 +175,6 @@ def trsm_cpu_baseline(L, B, budget_s=20.0):
             "sample": f"{done_rows} of {B.shape[0]} rows of B, n={n} ({t_total:.1f} s)"}
 
 
-def trsm_oracle_timing(L, B, rows=4):
-    """The oracle's scalar FP64 restatement of the solve, 1 core, on a few rows."""
-    import oracle
-    t0 = time.perf_counter()
-    oracle.dtrsm_rltn(L, B[:rows])
-    t = time.perf_counter() - t0
-    n = L.shape[0]
-    return {"value": round(rows * n * n / t / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
-            "kind": "port", "sample": f"{rows} rows of B, n={n} ({t:.1f} s)"}
-
-
 def run_trsm(args, torch, tensorblas):
     n, m, nb = args.n or 16384, args.m or 16384, args.nb
     log(f"bench: trsm inputs n={n} m={m} nb={nb}")
@@ -207,9 +204,17 @@ def run_trsm(args, torch, tensorblas):
     log("bench: plain FP16 tier")
     ms_plain = timed(h, step("plain"), 3, 1)
     Xp = dB.cpu().numpy().T[rows]
+    log("bench: split tier + FP64-residual refinement")
+
+    def refined():
+        dB.copy_(dB0)
+        h.trsm_refined("split", m, n, dL.data_ptr(), n, dB.data_ptr(), m, nb=nb)
+    ms_ref = timed(h, refined, 3, 1)
+    Xr = dB.cpu().numpy().T[rows]
     log("bench: FP64 host reference rows")
     R = trsm_host_rows(L, B, rows)
-    errs = {"split": err_metrics(Xs, R), "fp32": err_metrics(Xf, R), "plain": err_metrics(Xp, R)}
+    errs = {"split": err_metrics(Xs, R), "fp32": err_metrics(Xf, R), "plain": err_metrics(Xp, R),
+            "split_refined": err_metrics(Xr, R)}
 
     gemm_prof = {k: v for k, v in prof.items() if k.startswith("gemm_")}
     products = SPLIT_PRODUCTS[args.split_planes]
@@ -233,6 +238,9 @@ def run_trsm(args, torch, tensorblas):
         "vs_plain_fp32_fma": {"fp32_fma_tflops": round(flops / (ms_fp32 * 1e-3) / 1e12, 2),
                               "ratio": round(ms_fp32 / ms_split, 3)},
         "plain_fp16_tflops": round(flops / (ms_plain * 1e-3) / 1e12, 2),
+        "split_refined": {"tflops": round(flops / (ms_ref * 1e-3) / 1e12, 2), "ms": round(ms_ref, 3),
+                          "note": "tb_trsm_refined: split solve, FP64-MFMA residual B - X L^T, "
+                                  "split solve of the correction (flops counted once, m n^2)"},
         "errors_vs_fp64": errs,
         "roofline": roof,
         "roofline_kernel": rk,
@@ -240,16 +248,22 @@ def run_trsm(args, torch, tensorblas):
     })
     if not args.no_cpu:
         log("bench: host OpenBLAS strsm sample")
-        line["cpu_baseline"] = trsm_cpu_baseline(L, B)
-        log("bench: oracle restatement sample")
-        line["cpu_oracle"] = trsm_oracle_timing(L, B)
+        line["cpu_baseline"] = trsm_cpu_baseline(L, B, budget_s=10.0)
     h.close()
+    del dL, dB0, dB
+    torch.cuda.empty_cache()
     return line
 
 
 # ----------------------------------------------------------------------- SYRK
 def run_syrk(args, torch, tensorblas):
+    """SYRK C = A A^T (lower, alpha 1, beta 0) in the plain, split and FP32-FMA tiers.
+    n <= 4096: errors over the whole lower triangle against FP64 dsyrk; larger n: over
+    32 sampled columns (lower part) against FP64 products.  value = the `--tier` tier
+    (default plain at n <= 4096, split above); the roofline is that tier's GEMM."""
     n, k, nb = args.n or 2048, args.k or (args.n or 2048), args.nb
+    main = args.tier or ("plain" if n <= 4096 else "split")
+    log(f"bench: syrk n={n} k={k}")
     A = np.random.default_rng(0xC0F1).uniform(-1.0, 1.0, (n, k)).astype(np.float32)
     dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()     # column-major n x k
     dC = torch.zeros(n * n, device="cuda", dtype=torch.float32)
@@ -261,41 +275,69 @@ def run_syrk(args, torch, tensorblas):
         return lambda: h.syrk(tier, n, k, 1.0, dA.data_ptr(), n, 0.0, dC.data_ptr(), n, nb=nb)
 
     import scipy.linalg.blas as sb
-    R = sb.dsyrk(1.0, np.asfortranarray(A.astype(np.float64)), lower=1)
-    low = np.tril_indices(n)
+    if n <= 4096:
+        R = sb.dsyrk(1.0, np.asfortranarray(A.astype(np.float64)), lower=1)
+        low = np.tril_indices(n)
+
+        def err():
+            return err_metrics(dC.cpu().numpy().reshape(n, n).T[low], R[low])
+        ref_note = "FP64 OpenBLAS dsyrk, whole lower triangle"
+    else:
+        cols = np.sort(np.random.default_rng(13).choice(n, 32, replace=False))
+        A64 = A.astype(np.float64)
+        Rc = A64 @ np.ascontiguousarray(A64[cols].T)             # n x 32
+        del A64
+
+        def err():
+            Cv = dC.view(n, n)
+            got = np.concatenate([Cv[j, j:].cpu().numpy() for j in cols])
+            ref = np.concatenate([Rc[j:, c] for c, j in enumerate(cols)])
+            return err_metrics(got, ref)
+        ref_note = "FP64 products of 32 sampled columns (lower part)"
     out, errs = {}, {}
     for tier in ("plain", "split", "fp32"):
-        out[tier] = timed(h, step(tier), args.steps if tier == "plain" else 3, args.warmup)
-        Cg = dC.cpu().numpy().reshape(n, n).T
-        errs[tier] = err_metrics(Cg[low], R[low])
-    prof = profiled(h, step("plain"))
+        out[tier] = timed(h, step(tier), args.steps if tier == main else 3, args.warmup)
+        errs[tier] = err()
+    prof = profiled(h, step(main))
+    products = SPLIT_PRODUCTS[args.split_planes] if main == "split" else 1
     roof, rk = roofline({k_: v for k_, v in prof.items() if k_.startswith("gemm_")}, FP16_PEAK_TF,
-                        "TFLOP/s", "mfma", f"syrk{n}", "plain FP16 tier, one product per flop")
-    ms = out["plain"]
-    line = line_base("eff. TFLOP/s, plain FP16-MFMA SYRK", flops / (ms * 1e-3) / 1e12, "TFLOP/s",
-                     args.steps, args.warmup, ms, "f16 in, f32 accumulate",
+                        "TFLOP/s", "mfma", f"syrk{n}",
+                        f"{main} tier, {products} FP16 MFMA product(s) per algorithmic flop"
+                        + ("; one launch of a few microseconds: launch and latency bound" if n <= 2048 else ""))
+    if products > 1:
+        rk["mfma_executed_frac"] = round(roof["frac"] * products, 4)
+    ms = out[main]
+    line = line_base(f"eff. TFLOP/s, {main} SYRK", flops / (ms * 1e-3) / 1e12, "TFLOP/s",
+                     args.steps, args.warmup, ms,
+                     "f16 in, f32 accumulate" if main == "plain" else
+                     f"f32 ({args.split_planes}-plane fp16 split MFMA)",
                      "synthetic: A U[-1,1] FP32 (seeded)",
-                     {"workload": "SYRK C=A*A^T lower, alpha=1, beta=0", "n": n, "k": k, "nb": nb})
+                     {"workload": "SYRK C=A*A^T lower, alpha=1, beta=0", "n": n, "k": k, "nb": nb,
+                      "tier": main})
     line.update({
         "tiers_tflops": {t: round(flops / (v * 1e-3) / 1e12, 2) for t, v in out.items()},
         "split_vs_fp32_fma": round(out["fp32"] / out["split"], 3),
-        "errors_vs_fp64": errs, "roofline": roof, "roofline_kernel": rk,
+        "errors_vs_fp64": errs, "reference": ref_note, "roofline": roof, "roofline_kernel": rk,
         "profile_ms": {k_: round(v[1], 4) for k_, v in prof.items()},
     })
     if not args.no_cpu:
-        Af = np.asfortranarray(A)
-        sb.ssyrk(1.0, Af, lower=1)
+        ns = min(n, 8192)
+        Af = np.asfortranarray(A[:ns])
+        sb.ssyrk(1.0, Af[:256], lower=1)
         ts = []
-        for _ in range(3):
+        for _ in range(3 if ns <= 4096 else 1):
             t0 = time.perf_counter()
             sb.ssyrk(1.0, Af, lower=1)
             ts.append(time.perf_counter() - t0)
         t = float(np.median(ts))
-        line["cpu_baseline"] = {"value": round(flops / t / 1e12, 4), "unit": "TFLOP/s",
+        line["cpu_baseline"] = {"value": round(float(ns) * ns * k / t / 1e12, 4), "unit": "TFLOP/s",
                                 "cores": int(host_threads()), "kind": "port",
                                 "impl": "host OpenBLAS ssyrk via scipy.linalg.blas",
-                                "sample": f"full problem, median of 3 ({t * 1e3:.1f} ms)"}
+                                "sample": (f"full problem, median of 3 ({t * 1e3:.1f} ms)" if ns == n else
+                                           f"first {ns} rows of A (n={ns}, k={k}), one run ({t:.2f} s)")}
     h.close()
+    del dA, dC
+    torch.cuda.empty_cache()
     return line
 
 
@@ -428,77 +470,140 @@ def run_trtri(args, torch, tensorblas):
 
 
 # ---------------------------------------------------------------------- Ozaki
-def ozaki_inputs(n, k, phi, seed):
-    g = np.random.default_rng(seed)
+OZAKI_DEFAULT_SLICES = {0.5: "8,8+,9", 2.0: "8+,9+,10"}
 
-    def mat():
-        return (g.uniform(-0.5, 0.5, (n, k)) * np.exp(phi * g.standard_normal((n, k))))
-    return mat(), mat()
+
+def slice_products(S, extra):
+    """INT8 MFMA slice products per FP64 multiply-add: the triangular scheme's
+    S(S+1)/2, plus the S-1 pairs of the group g = S when extra."""
+    return S * (S + 1) // 2 + extra * (S - 1)
+
+
+def int_slices(X, bits, nslices):
+    """X[i, :] = sum_s S[s][i, :] * 2^(e[i] - bits*(s+1)) + (a remainder below
+    2^(e[i] - bits*nslices)), each S[s] an integer-valued float64 array with
+    |S[s]| < 2^bits, e[i] the exponent with max|X[i, :]| < 2^e[i].  Every step (scaling
+    by a power of two, truncation, subtraction of the integer part) is exact."""
+    mx = np.abs(X).max(axis=1)
+    e = np.where(mx > 0, np.frexp(mx)[1], 0).astype(np.int64)
+    R = X * np.ldexp(1.0, bits - e)[:, None]
+    out = []
+    for _ in range(nslices):
+        a = np.trunc(R)
+        out.append(a)
+        R -= a
+        R *= 2.0 ** bits
+    return out, e
+
+
+def exact_rowdot(pairs, cols):
+    """out[:, c] = sum over (X, Y) in pairs of X @ Y[cols[c]], to within a few FP64 ulps:
+    rows of X and the sampled rows of Y are cut into three integer slices of b bits
+    (3b >= 57) with 2b + log2(k) <= 53, so every FP64 BLAS product of two slices sums
+    integers below 2^53 exactly; the nine (eighteen) exact slice products are scaled
+    and summed in long double and rounded once."""
+    k = pairs[0][0].shape[1]
+    bits = (53 - int(np.ceil(np.log2(max(k, 2))))) // 2
+    acc = None
+    for X, Y in pairs:
+        xs, ex = int_slices(X, bits, 3)
+        ys, ey = int_slices(Y[cols], bits, 3)
+        for s_, xa in enumerate(xs):
+            for t_, yb in enumerate(ys):
+                G = (xa @ yb.T).astype(np.longdouble)                       # exact integers
+                sc = np.ldexp(np.longdouble(1.0), (ex[:, None] + ey[None, :] - bits * (s_ + t_ + 2)))
+                acc = G * sc if acc is None else acc + G * sc
+        del xs, ys
+    return acc.astype(np.float64)
 
 
 def run_ozaki(args, torch, tensorblas):
     """SYR2K C = A B^T + B A^T (n x k operands) or SYMM C = A B (A m x m symmetric,
     lower stored; B m x n), FP64 by Ozaki INT8 slicing at several slice counts and by
-    native FP64 MFMA; error against an extended-precision host product of sampled
-    columns."""
+    native FP64 MFMA.  Entries U[-0.5,0.5]*exp(phi*N(0,1)), generated on the device from
+    a seeded generator.  Errors: max-relative and normwise over 64 sampled output columns
+    against exact_rowdot's products of the same columns (exact integer slice products
+    on FP64 BLAS, accurate to a few ulps; a plain FP64 product would itself be off by
+    about 1e-15 of max|C| here, too close to the 1e-14 target to decide it)."""
     op = args.op or "syr2k"
     n, k = args.n or 32768, args.k or 8192
     phi = args.phi
-    g = np.random.default_rng(0xC0F4 if op == "syr2k" else 0xC0F5)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed((0xC0F4 if op == "syr2k" else 0xC0F5) + int(round(phi * 10)))
 
     def mat(r, c):
-        return g.uniform(-0.5, 0.5, (r, c)) * np.exp(phi * g.standard_normal((r, c)))
-    cols = np.sort(np.random.default_rng(11).choice(n if op == "syr2k" else k, 4, replace=False))
+        u = torch.rand(r, c, device="cuda", dtype=torch.float64, generator=gen) - 0.5
+        return u.mul_(torch.randn(r, c, device="cuda", dtype=torch.float64, generator=gen).mul_(phi).exp_())
+    ncols = 64
+    cols = np.sort(np.random.default_rng(11).choice(n if op == "syr2k" else k, ncols, replace=False))
+    log(f"bench: ozaki {op} n={n} k={k} phi={phi}")
     if op == "syr2k":
-        A, B = mat(n, k), mat(n, k)
-        dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()     # column-major n x k
-        dB = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+        # tensor rows are the columns of the column-major n x k operands
+        dA, dB = mat(k, n), mat(k, n)
         dC = torch.zeros(n * n, device="cuda", dtype=torch.float64)
         flops = 2.0 * n * n * k
-        Al, Bl = A.astype(np.longdouble), B.astype(np.longdouble)
-        # C(i, j) = A(i,:).B(j,:) + B(i,:).A(j,:), lower part i >= j
-        ref = {int(j): (Al[j:] @ Bl[j] + Bl[j:] @ Al[j]).astype(np.float64) for j in cols}
-        del Al, Bl
+        A = dA.cpu().numpy().T          # n x k views (F order)
+        B = dB.cpu().numpy().T
+        # C(:, j) = A B(j,:)^T + B A(j,:)^T, all rows (lower part compared)
+        ref = exact_rowdot([(np.ascontiguousarray(A), np.ascontiguousarray(B)),
+                            (np.ascontiguousarray(B), np.ascontiguousarray(A))], cols)
 
         def step(tier):
             return lambda: h.syr2k(tier, n, k, 1.0, dA.data_ptr(), n, dB.data_ptr(), n, 0.0,
                                    dC.data_ptr(), n, nb=256)
 
-        def out_col(j):
-            return dC.view(n, n)[j, j:].cpu().numpy()
+        def out_col(c, j):
+            return dC.view(n, n)[j, j:].cpu().numpy(), ref[j:, c]
         shape = {"n": n, "k": k}
+        sample = 8192
+
+        def host_leg():
+            import scipy.linalg.blas as sb
+            As, Bs = np.asfortranarray(A[:sample]), np.asfortranarray(B[:sample])
+            sb.dsyr2k(1.0, As[:512], Bs[:512], lower=1)
+            t0 = time.perf_counter()
+            sb.dsyr2k(1.0, As, Bs, lower=1)
+            t = time.perf_counter() - t0
+            return 2.0 * sample * sample * k / t / 1e12, t, f"dsyr2k n={sample} (first rows), k={k}"
     else:
         m, nn = n, k
-        L = np.tril(mat(m, m))
-        A = L + np.tril(L, -1).T                                      # symmetric
-        del L
-        B = mat(m, nn)
-        dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()      # column-major m x m
-        dB = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()      # column-major m x n
+        dA = mat(m, m)                    # column-major m x m: only the lower triangle is read
+        dB = mat(nn, m)                   # column-major m x nn
         dC = torch.zeros(m * nn, device="cuda", dtype=torch.float64)
         flops = 2.0 * m * m * nn
-        ref = {}
-        for j in cols:
-            bj = B[:, j].astype(np.longdouble)
-            acc = np.zeros(m, dtype=np.longdouble)
-            for r0 in range(0, m, 2048):                           # bounded host memory
-                acc[r0:r0 + 2048] = A[r0:r0 + 2048].astype(np.longdouble) @ bj
-            ref[int(j)] = acc.astype(np.float64)
-        del A
+        Mc = dA.cpu().numpy()             # Mc[j, i] = A(i, j): A(i, j) = Mc[min, max]
+        B = dB.cpu().numpy().T            # m x nn
+        # rows of the symmetric A: A(i, j) = Mc[min(i,j), max(i,j)]
+        Asym = np.triu(Mc)
+        Asym += np.triu(Mc, 1).T
+        ref = exact_rowdot([(Asym, np.ascontiguousarray(B.T))], cols)
+        del Asym
 
         def step(tier):
             return lambda: h.symm(tier, m, nn, 1.0, dA.data_ptr(), m, dB.data_ptr(), m, 0.0,
                                   dC.data_ptr(), m)
 
-        def out_col(j):
-            return dC.view(nn, m)[j].cpu().numpy()
+        def out_col(c, j):
+            return dC.view(nn, m)[j].cpu().numpy(), ref[:, c]
         shape = {"m": m, "n": nn}
+        sample = 8192
+
+        def host_leg():
+            import scipy.linalg.blas as sb
+            Asub = np.asfortranarray(np.triu(Mc[:sample, :sample]).T)   # lower-stored block
+            Bsub = np.asfortranarray(B[:sample, :sample])
+            sb.dsymm(1.0, Asub[:512, :512], Bsub[:512, :512], side=0, lower=1)
+            t0 = time.perf_counter()
+            sb.dsymm(1.0, Asub, Bsub, side=0, lower=1)
+            t = time.perf_counter() - t0
+            return 2.0 * sample * sample * sample / t / 1e12, t, f"dsymm m={sample} n={sample} (leading block)"
     h = tensorblas.Handle()
 
     def col_err():
         worst, num, den = 0.0, 0.0, 0.0
-        for j, r in ref.items():
-            d = np.abs(out_col(j) - r)
+        for c, j in enumerate(cols):
+            got, r = out_col(c, j)
+            d = np.abs(got - r)
             worst = max(worst, float(d.max() / np.abs(r).max()))
             num += float((d ** 2).sum())
             den += float((r ** 2).sum())
@@ -506,14 +611,17 @@ def run_ozaki(args, torch, tensorblas):
 
     res = {}
     # "8" = 8 slices, triangular scheme; "8+" = 8 slices plus the slice-pair group g = 8
-    for tok in args.slices.split(","):
-        s, extra = int(tok.rstrip("+")), int(tok.endswith("+"))
-        h.set_ozaki_slices(s)
+    slices = args.slices or OZAKI_DEFAULT_SLICES.get(phi, "8,8+,9")
+    for tok in slices.split(","):
+        s_, extra = int(tok.rstrip("+")), int(tok.endswith("+"))
+        h.set_ozaki_slices(s_)
         h.set_ozaki_extra(extra)
         ms = timed(h, step("ozaki"), args.steps, args.warmup)
         res[f"ozaki_s{tok}"] = {"ms": round(ms, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                                "int8_products": slice_products(s_, extra),
                                 "errors_vs_ref": col_err()}
-    ms64 = timed(h, step("fp64"), 2, 1)
+        log(f"bench: ozaki {op} phi={phi} S={tok}: {res[f'ozaki_s{tok}']}")
+    ms64 = timed(h, step("fp64"), args.steps, args.warmup)
     res["fp64_mfma"] = {"ms": round(ms64, 2), "tflops": round(flops / (ms64 * 1e-3) / 1e12, 2),
                         "errors_vs_ref": col_err()}
     ok = [(v["ms"], key) for key, v in res.items()
@@ -524,23 +632,34 @@ def run_ozaki(args, torch, tensorblas):
     h.set_ozaki_slices(sbest)
     h.set_ozaki_extra(xbest)
     prof = profiled(h, step("ozaki"))
-    oz = {k_: v for k_, v in prof.items() if k_ == "ozaki_gemm"}
-    roof, rk = roofline(oz, FP64_PEAK_TF, "TFLOP/s", "mfma", f"{op}{n}x{k}",
-                        f"FP64-equivalent flops; {sbest} slices{' + group g=S' if xbest else ''} run "
-                        f"{sbest * (sbest + 1) // 2 + xbest * (sbest - 1)} "
-                        f"INT8 MFMA slice products per flop (INT8 peak {INT8_PEAK_TOPS} TOPS)")
+    products = slice_products(sbest, xbest)
+    oz = {k_: (v[0], v[1], v[2] * products) for k_, v in prof.items() if k_ == "ozaki_gemm"}
+    roof, rk = roofline(oz, INT8_PEAK_TOPS, "TOPS", "mfma", f"{op}{n}x{k}_s{tbest}",
+                        f"INT8 MFMA operations = FP64-equivalent flops x {products} slice products "
+                        f"({sbest} slices{' + group g=S' if xbest else ''}); the FP64-equivalent "
+                        f"ceiling is {INT8_PEAK_TOPS / products:.1f} TFLOP/s")
+    rk["fp64_equiv_tflops"] = round(roof["achieved"] / products, 2)
     ms = best[0]
     line = line_base(f"eff. FP64 TFLOP/s of Ozaki {op.upper()} at <=1e-14 max relative error",
                      flops / (ms * 1e-3) / 1e12, "TFLOP/s", args.steps, args.warmup, ms,
                      "f64 (int8 slices, fp64 accumulate)",
-                     f"synthetic: U[-0.5,0.5]*exp({phi}*N(0,1)) FP64 (seeded)",
+                     f"synthetic: U[-0.5,0.5]*exp({phi}*N(0,1)) FP64 (seeded, generated on the device)",
                      {"workload": f"{op.upper()}, Ozaki INT8 slicing", **shape, "phi": phi,
                       "slices": sbest, "extra_group": xbest})
+    e64 = res["fp64_mfma"]["errors_vs_ref"]["maxrel"]
     line.update({"results": res, "vs_fp64_mfma": round(ms64 / ms, 3), "meets_1e-14": bool(ok),
+                 "fp64_mfma_meets_1e-14": bool(e64 <= 1e-14),
                  "roofline": roof, "roofline_kernel": rk,
                  "profile_ms": {k_: round(v[1], 3) for k_, v in prof.items()},
-                 "reference": "numpy longdouble products on 4 sampled columns"})
+                 "reference": f"exact integer-slice products of {ncols} sampled output columns"})
+    if not args.no_cpu:
+        tf, t, what = host_leg()
+        line["cpu_baseline"] = {"value": round(tf, 4), "unit": "TFLOP/s", "cores": int(host_threads()),
+                                "kind": "port", "impl": "host OpenBLAS via scipy.linalg.blas (FP64)",
+                                "sample": f"{what}, one run ({t:.2f} s)"}
     h.close()
+    del dA, dB, dC
+    torch.cuda.empty_cache()
     return line
 
 
@@ -668,7 +787,7 @@ def run_batched(args, torch, tensorblas):
             ts += time.perf_counter() - t0
             fl += float(n) ** 3
             done += 1
-            if ts > 10.0:
+            if ts > args.cpu_budget:
                 break
         line["cpu_baseline"] = {"value": round(fl / ts / 1e12, 4), "unit": "TFLOP/s",
                                 "cores": int(host_threads()), "kind": "port",
@@ -676,6 +795,8 @@ def run_batched(args, torch, tensorblas):
                                 "sample": f"first {done} of {count} instances ({ts:.1f} s)"}
     plan.close()
     h.close()
+    del ins, bout, bsave, cout, views
+    torch.cuda.empty_cache()
     return line
 
 
@@ -854,6 +975,41 @@ def run_copy(args, torch, tensorblas):
                      {"workload": "256 MiB device-to-device copy"})
 
 
+# ------------------------------------------------------- the default bench line
+# The other configurations of the benchmark, each measured in the same process after
+# the headline TRSM and carried in the line under "configs": (key, workload, options).
+DEFAULT_LEGS = [
+    ("config1_syrk2048", "syrk", dict(n=2048, k=2048, tier="plain", steps=50, warmup=5)),
+    ("syrk16384_split_vs_fp32", "syrk", dict(n=16384, k=16384, tier="split", steps=5, warmup=1)),
+    ("config3_ozaki_syr2k_phi0.5", "ozaki", dict(op="syr2k", phi=0.5, steps=2, warmup=1)),
+    ("config3_ozaki_syr2k_phi2", "ozaki", dict(op="syr2k", phi=2.0, steps=2, warmup=1)),
+    ("config3_ozaki_symm_phi0.5", "ozaki", dict(op="symm", phi=0.5, steps=2, warmup=1)),
+    ("config4_batched2000", "batched", dict(count=2000, tier="split", steps=10, warmup=2)),
+]
+LEG_DROP = ("n_gpus", "scaling", "vs_baseline", "higher_is_better")
+
+
+def run_leg(fn, args, torch, tensorblas, **over):
+    """One configuration of the default line: the workload's own line, or the error it
+    raised (so that one failing leg does not hide the others)."""
+    import copy
+    a = copy.copy(args)
+    for key in ("n", "m", "k", "count"):
+        setattr(a, key, 0)
+    a.tier, a.op, a.slices = "", "", ""
+    for key, v in over.items():
+        setattr(a, key, v)
+    t0 = time.perf_counter()
+    try:
+        line = fn(a, torch, tensorblas)
+        out = {k_: v for k_, v in line.items() if k_ not in LEG_DROP}
+    except Exception as e:       # a library error status or a host-side failure
+        out = {"error": f"{type(e).__name__}: {e}"}
+    out["leg_seconds"] = round(time.perf_counter() - t0, 1)
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -882,13 +1038,19 @@ def main():
     ap.add_argument("--k", type=int, default=0)
     ap.add_argument("--nb", type=int, default=256)
     ap.add_argument("--tiers", default="plain,split,fp32")
-    ap.add_argument("--slices", default="6,7,8,9")
+    ap.add_argument("--slices", default="",
+                    help="ozaki: slice counts to time, '+' = with the group g = S "
+                         "(default: 8,8+,9 at phi 0.5; 8+,9+,10 at phi 2)")
     ap.add_argument("--phi", type=float, default=0.5)
     ap.add_argument("--op", default="", choices=["", "syr2k", "symm"])
     ap.add_argument("--split-planes", type=int, default=2, choices=[2, 3])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true",
                     help="trsm: skip the request-stream summary the default line carries")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="trsm: skip the other configurations the default line carries")
+    ap.add_argument("--cpu-budget", type=float, default=5.0,
+                    help="batched: seconds of host BLAS for its cpu_baseline sample")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1")
@@ -907,6 +1069,12 @@ def main():
            "copy": run_copy, "batched": run_batched, "serve": run_serve,
            "potrf": run_potrf, "trtri": run_trtri}[args.workload]
     line = run(args, torch, tensorblas)
+    if args.workload == "trsm" and not args.no_configs:
+        fns = {"syrk": run_syrk, "ozaki": run_ozaki, "batched": run_batched}
+        line["configs"] = {}
+        for key, wl, over in DEFAULT_LEGS:
+            log(f"bench: configuration {key}")
+            line["configs"][key] = run_leg(fns[wl], args, torch, tensorblas, **over)
     if with_serve:
         # the metric's second half: the request stream (--workload serve, default
         # policy) in brief, measured after the TRSM in the same process
@@ -916,6 +1084,7 @@ def main():
             "workload": sv["config"]["workload"],
             "latency_ms": sv["latency_ms"], "isolated_latency_ms": sv["isolated_latency_ms"],
             "p99_latency_vs_isolated_p99": sv["p99_latency_vs_isolated_p99"],
+            "latency_over_isolated": sv["latency_over_isolated"],
             "throughput_tflops": round(sv["value"], 3),
             "throughput_vs_offline": sv["throughput_vs_offline"],
             "results_equal_alone": sv["results_equal_alone"],
