@@ -159,15 +159,17 @@ TB_API int tb_trsm_refined(tb_handle_t h, int tier, long m, long n, const void* 
  * in the reference, which stops at the BLAS-3 routines; this is their canonical
  * caller, composed from them: recursive halving, A21 := A21 L11^-T by tb_trsm and
  * A22 -= A21 A21^T by tb_syrk in the given tier, diagonal blocks of at most 128
- * factored in FP64 on the device.  *info = 0, or j > 0 when the leading minor of order
+ * factored on the device in the matrix's precision (FP32 for plain/split/fp32, FP64 for
+ * ozaki/fp64).  *info = 0, or j > 0 when the leading minor of order
  * j is not positive definite (LAPACK dpotrf's info; A then holds a partial factor).
  * Synchronizes the handle's stream to read info.  float* A for plain/split/fp32,
  * double* A for ozaki/fp64; nb is passed to tb_trsm and tb_syrk. */
 TB_API int tb_potrf(tb_handle_t h, int tier, long n, void* A, long lda, long nb, long* info);
 /* Triangular inverse in place, A := inv(A), for the uplo triangle of A (no counterpart in
  * the reference; LAPACK dtrtri's contract): uplo 'L'/'U', diag 'N' or 'U' (unit diagonal,
- * not referenced).  Composed from this library's TRSM in the chosen tier, with FP64
- * inverses of the diagonal blocks; the other strict triangle is not changed.  *info = 0,
+ * not referenced).  Composed from this library's TRMM in the chosen tier, with the
+ * diagonal blocks inverted in the matrix's precision; the other strict triangle is not
+ * changed.  *info = 0,
  * or i when A(i, i) is exactly zero (A is then left unchanged).  n^3/3 flops.
  * Synchronizes the handle's stream (diag 'N': to read the zero test). */
 TB_API int tb_trtri(tb_handle_t h, int tier, char uplo, char diag, long n, void* A, long lda,

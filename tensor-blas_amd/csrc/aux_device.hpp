@@ -87,15 +87,16 @@ __device__ __forceinline__ void rowexp_k_block(const T* __restrict__ X, long lon
 
 // Diagonal block L (s x s, lower) = (I + N) D with D = diag(L): inv(L) = D^-1 (I+N)^-1.
 // Writes W = D inv(L) - I = (I+N)^-1 - I (strictly lower) as Z = W^T (Z[c + r*s] =
-// W(r, c)) and dinv[r] = 1/L(r,r), FP64 arithmetic.  One wave owns the kTrtriCols
+// W(r, c)) and ldiag[r] = L(r,r), in the arithmetic of T: FP32 for the FP32-class tiers
+// (plain, split, FP32-FMA), FP64 for the FP64 tiers.  One wave owns the kTrtriCols
 // columns cb.. of W and solves them right-looking: lane l holds rows l, l+64, l+128,
 // l+192 of those columns in registers (acc, starting as the identity's columns); at
-// step r the finished row r is broadcast by readlane, v = acc_r / L(r,r), and every
-// later row q subtracts L(q,r) v.  Row q's acc is final once step q is reached and is
-// W(q, .) itself (W = D inv(L) - I needs no final scale), so the wave stores it at the
-// end.  Column r of L is contiguous in memory and is loaded kTrtriAhead steps early
-// into a register ring.  No LDS and no barriers: the chain per step is one readlane
-// broadcast, one division and the row updates (s <= 256).
+// step r the finished row r is broadcast by readlane (a wavefront shuffle), v = acc_r /
+// L(r,r), and every later row q subtracts L(q,r) v.  Row q's acc is final once step q
+// is reached and is W(q, .) itself (W = D inv(L) - I needs no final scale), so the wave
+// stores it at the end.  Column r of L is contiguous in memory and is loaded
+// kTrtriAhead steps early into a register ring.  No LDS and no barriers: the chain per
+// step is one readlane broadcast, one division and the row updates (s <= 256).
 constexpr int kTrtriAhead = 8;
 
 __device__ __forceinline__ double trtri_readlane(double x, int lane)
@@ -106,26 +107,30 @@ __device__ __forceinline__ double trtri_readlane(double x, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <typename Tin, typename Tout>
-__device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long long lda, int s,
-                                            Tout* __restrict__ Zq, double* __restrict__ dinvq,
-                                            int cb)
+__device__ __forceinline__ float trtri_readlane(float x, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+template <typename T>
+__device__ __forceinline__ void trtri_block(const T* __restrict__ Lq, long long lda, int s,
+                                            T* __restrict__ Zq, T* __restrict__ ldiagq, int cb)
 {
     constexpr int C = kTrtriCols;
     const int lane = threadIdx.x & 63;
     if (cb == 0)
-        for (int r = lane; r < s; r += 64) dinvq[r] = 1.0 / (double)Lq[r + (long long)r * lda];
-    double acc[4][C];
+        for (int r = lane; r < s; r += 64) ldiagq[r] = Lq[r + (long long)r * lda];
+    T acc[4][C];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int c = 0; c < C; ++c) acc[u][c] = (u * 64 + lane == cb + c) ? 1.0 : 0.0;
+        for (int c = 0; c < C; ++c) acc[u][c] = (u * 64 + lane == cb + c) ? T(1) : T(0);
     // column r of L at rows u*64 + lane >= r (others read as zero), and 1 / L(r, r)
     // (broadcast from the lane holding the diagonal; the division is off the chain)
-    double pre[kTrtriAhead][4], rcpr[kTrtriAhead];
+    T pre[kTrtriAhead][4], rcpr[kTrtriAhead];
     // unconditional loads at clamped addresses (masked_load), so a step's loads never
     // wait for each other
-    auto fetch = [&](int r, double* v) {
+    auto fetch = [&](int r, T* v) {
         const int rc = min(r, s - 1);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -133,12 +138,12 @@ __device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long lon
             v[u] = masked_load(Lq + qc + (long long)rc * lda, r < s && q < s && q >= r);
         }
     };
-    auto diag_rcp = [&](int r, const double* v) {
+    auto diag_rcp = [&](int r, const T* v) {
         const int ur = (r >> 6) & 3, lr = r & 63;
-        double dg = trtri_readlane(v[0], lr);
+        T dg = trtri_readlane(v[0], lr);
 #pragma unroll
         for (int u = 1; u < 4; ++u) dg = ur == u ? trtri_readlane(v[u], lr) : dg;
-        return r < s ? 1.0 / dg : 0.0;
+        return r < s ? T(1) / dg : T(0);
     };
 #pragma unroll
     for (int d = 0; d < kTrtriAhead; ++d) {
@@ -155,13 +160,13 @@ __device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long lon
                 const int lr = lr0 + d;
                 if (lr < last) {
                     const int r = u * 64 + lr;
-                    const double rcp = rcpr[d];                                  // 1 / L(r, r)
-                    double v[C];
+                    const T rcp = rcpr[d];                                       // 1 / L(r, r)
+                    T v[C];
 #pragma unroll
                     for (int c = 0; c < C; ++c) v[c] = trtri_readlane(acc[u][c], lr) * rcp;
 #pragma unroll
                     for (int w = u; w < 4; ++w) {
-                        const double lq = pre[d][w];             // L(w*64 + lane, r), 0 unless below r
+                        const T lq = pre[d][w];                  // L(w*64 + lane, r), 0 unless below r
                         if (w > u || lane > lr) {
 #pragma unroll
                             for (int c = 0; c < C; ++c) acc[w][c] -= lq * v[c];
@@ -180,7 +185,7 @@ __device__ __forceinline__ void trtri_block(const Tin* __restrict__ Lq, long lon
         if (q < s) {
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                if (cb + c < s) Zq[(cb + c) + (long long)q * s] = (q > cb + c) ? (Tout)acc[u][c] : Tout(0);
+                if (cb + c < s) Zq[(cb + c) + (long long)q * s] = (q > cb + c) ? acc[u][c] : T(0);
         }
     }
 }

@@ -36,26 +36,26 @@ __global__ void fill_i32_kernel(int* p, long long n, int v)
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) p[i] = v;
 }
 
-template <typename Tin, typename Tout>
+template <typename T>
 __global__ void __launch_bounds__(64)
-trtri_lower_t_kernel(const Tin* __restrict__ L, long long lda, long long bstride, int s,
-                     Tout* __restrict__ Z, double* __restrict__ dinv)
+trtri_lower_t_kernel(const T* __restrict__ L, long long lda, long long bstride, int s,
+                     T* __restrict__ Z, T* __restrict__ ldiag)
 {
-    trtri_block(L + (long long)blockIdx.y * bstride, lda, s, Z + (long long)blockIdx.y * s * s,
-                dinv + (long long)blockIdx.y * s, blockIdx.x * kTrtriCols);
+    trtri_block<T>(L + (long long)blockIdx.y * bstride, lda, s, Z + (long long)blockIdx.y * s * s,
+                   ldiag + (long long)blockIdx.y * s, blockIdx.x * kTrtriCols);
 }
 
-// one column c of the block per workgroup: A(r, c) = dinv[r] (delta(r, c) + W(r, c)),
-// W(r, c) = Z[c + r*s] (Z = W^T)
+// one column c of the block per workgroup: A(r, c) = (delta(r, c) + W(r, c)) / L(r, r),
+// W(r, c) = Z[c + r*s] (Z = W^T), in T's arithmetic
 template <typename T>
 __global__ void __launch_bounds__(256)
-trtri_store_kernel(const double* __restrict__ Z, const double* __restrict__ dinv, int s,
+trtri_store_kernel(const T* __restrict__ Z, const T* __restrict__ ldiag, int s,
                    T* __restrict__ A, long long lda, int unit)
 {
     const int c = blockIdx.x;
     for (int r = c + unit + (int)threadIdx.x; r < s; r += 256) {
-        const double w = r == c ? 1.0 : Z[c + (long long)r * s];
-        A[r + (long long)c * lda] = (T)(dinv[r] * w);
+        const T w = r == c ? T(1) : Z[c + (long long)r * s];
+        A[r + (long long)c * lda] = w / ldiag[r];
     }
 }
 
@@ -233,38 +233,31 @@ hipError_t rowexp(bool f64, const void* X, long long ld, int mode, int rows, int
     return hipSuccess;
 }
 
-hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda,
-                         long long bstride, int s, int nblocks, void* Z, double* dinv,
-                         hipStream_t st)
+hipError_t trtri_lower_t(bool f64, const void* L, long long lda, long long bstride, int s,
+                         int nblocks, void* Z, void* ldiag, hipStream_t st)
 {
     if (s <= 0 || nblocks <= 0) return hipSuccess;
     if (s > 256) return hipErrorInvalidValue;
     dim3 g((s + kTrtriCols - 1) / kTrtriCols, nblocks);
-    if (in_f64 && out_f64)
-        hipLaunchKernelGGL((trtri_lower_t_kernel<double, double>), g, dim3(64), 0, st,
-                           (const double*)L, lda, bstride, s, (double*)Z, dinv);
-    else if (in_f64)
-        hipLaunchKernelGGL((trtri_lower_t_kernel<double, float>), g, dim3(64), 0, st,
-                           (const double*)L, lda, bstride, s, (float*)Z, dinv);
-    else if (out_f64)
-        hipLaunchKernelGGL((trtri_lower_t_kernel<float, double>), g, dim3(64), 0, st,
-                           (const float*)L, lda, bstride, s, (double*)Z, dinv);
+    if (f64)
+        hipLaunchKernelGGL((trtri_lower_t_kernel<double>), g, dim3(64), 0, st,
+                           (const double*)L, lda, bstride, s, (double*)Z, (double*)ldiag);
     else
-        hipLaunchKernelGGL((trtri_lower_t_kernel<float, float>), g, dim3(64), 0, st,
-                           (const float*)L, lda, bstride, s, (float*)Z, dinv);
+        hipLaunchKernelGGL((trtri_lower_t_kernel<float>), g, dim3(64), 0, st,
+                           (const float*)L, lda, bstride, s, (float*)Z, (float*)ldiag);
     return hipGetLastError();
 }
 
-hipError_t trtri_store(bool f64, const double* Z, const double* dinv, int s, void* A, long long lda,
+hipError_t trtri_store(bool f64, const void* Z, const void* ldiag, int s, void* A, long long lda,
                        bool unit, hipStream_t st)
 {
     if (s <= 0) return hipSuccess;
     if (f64)
-        hipLaunchKernelGGL(trtri_store_kernel<double>, dim3(s), dim3(256), 0, st, Z, dinv, s,
-                           (double*)A, lda, (int)unit);
+        hipLaunchKernelGGL(trtri_store_kernel<double>, dim3(s), dim3(256), 0, st,
+                           (const double*)Z, (const double*)ldiag, s, (double*)A, lda, (int)unit);
     else
-        hipLaunchKernelGGL(trtri_store_kernel<float>, dim3(s), dim3(256), 0, st, Z, dinv, s,
-                           (float*)A, lda, (int)unit);
+        hipLaunchKernelGGL(trtri_store_kernel<float>, dim3(s), dim3(256), 0, st,
+                           (const float*)Z, (const float*)ldiag, s, (float*)A, lda, (int)unit);
     return hipGetLastError();
 }
 

@@ -159,6 +159,7 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
         P.eOut = I.eRow;
         P.ldc = I.ldc;
         P.beta = 1.0;
+        P.beta32 = 1.0f;
         if (st.type == 0) {
             // leaf: B(:, off:off+s) := (B + B W^T) D^-1 in place (one tile spans all s columns)
             const int off = st.a, s = st.b;
@@ -175,7 +176,8 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
             P.srcA = P.srcB = kPP;
             P.C = I.C + (long long)off * I.ldc;
             P.alpha = 1.0;
-            P.cscale = I.dinv + off;
+            P.alpha32 = 1.0f;
+            P.cdiv = I.ldiag + off;
             P.eB = I.eZ + st.eoff;
             panel_tile<ANPL, kPP, NW>(P, i0, 0, raw);
             continue;
@@ -187,6 +189,7 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
         P.k = kw;
         P.C = I.C + (long long)c0 * I.ldc;
         P.alpha = -1.0;
+        P.alpha32 = -1.0f;
         P.eB = I.eL + c0;
         {
             // L's plane rows give 16-byte aligned 32-k stages from column k0 (in the copy
@@ -249,6 +252,8 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
             P.ldc = I.ldc;
             P.alpha = I.alpha;
             P.beta = I.beta;
+            P.alpha32 = I.alpha;
+            P.beta32 = I.beta;
             P.modeC = kCLower;
             P.eA = P.eB = I.eRow;
             P.batch = 1;
@@ -291,8 +296,8 @@ batch_trtri_kernel(const TrtriJob* __restrict__ jobs, int njobs)
     const TrtriJob& J = find_job(jobs, njobs, blockIdx.x);
     const int local = blockIdx.x - J.blockBegin;
     const int q = local / J.colBlocks, cb = (local % J.colBlocks) * kTrtriCols;
-    trtri_block(J.L + (long long)q * J.bstride, J.lda, J.s, J.Z + (long long)q * J.s * J.s,
-                J.dinv + (long long)q * J.s, cb);
+    trtri_block<float>(J.L + (long long)q * J.bstride, J.lda, J.s, J.Z + (long long)q * J.s * J.s,
+                       J.ldiag + (long long)q * J.s, cb);
 }
 
 __global__ void __launch_bounds__(256)

@@ -31,6 +31,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "epilogue.hpp"
 #include "gemm_kernel.hpp"   // vector types, xcd_remap, kExpZeroDev
 #include "tb_common.hpp"
 
@@ -485,7 +486,7 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     }
 
     // ----------------------------------------------------------------- epilogue
-    const float alpha = (float)P.alpha, beta = (float)P.beta;
+    const float alpha = P.alpha32, beta = P.beta32;
     const bool lowerC = (P.modeC == kCLower);
     const bool track = (P.eOut != nullptr);
     const long long ldc = P.ldc;
@@ -508,15 +509,16 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
             for (int hf = 0; hf < 16 / G; ++hf) {
                 bool in[G];
                 float cold[G];
-                double cs[G];
+                float cd[G];
+                const float* cdiv = static_cast<const float*>(P.cdiv);
 #pragma unroll
                 for (int u = 0; u < G; ++u) {
                     const int r = hf * G + u;
                     const int j = j0 + wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                     in[u] = i < M && j < N && (!lowerC || i >= j);
-                    cold[u] = (float)masked_load(Cb + (in[u] ? (long long)i + (long long)j * ldc : 0ll),
-                                                 in[u] && beta != 0.0f);
-                    cs[u] = P.cscale ? masked_load(P.cscale + (in[u] ? j : 0), in[u]) : 1.0;
+                    cold[u] = masked_load(Cb + (in[u] ? (long long)i + (long long)j * ldc : 0ll),
+                                          in[u] && beta != 0.0f);
+                    cd[u] = cdiv ? masked_load(cdiv + (in[u] ? j : 0), in[u]) : 1.0f;
                 }
 #pragma unroll
                 for (int u = 0; u < G; ++u) {
@@ -532,9 +534,8 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
                     if (in[u]) {
                         float* c = Cb + (long long)i + (long long)j * ldc;
                         float w;
-                        if (P.cscale) {
-                            const double co = (beta == 0.0f) ? 0.0 : (double)cold[u];
-                            w = (float)(((double)alpha * (double)v + (double)beta * co) * cs[u]);
+                        if (cdiv) {
+                            w = div_sum_f32(alpha * v, beta == 0.0f ? 0.0f : beta * cold[u], cd[u]);
                         } else {
                             w = (beta == 0.0f) ? alpha * v : alpha * v + beta * cold[u];
                         }

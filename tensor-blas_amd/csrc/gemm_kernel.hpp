@@ -21,6 +21,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "epilogue.hpp"
 #include "masked_load.hpp"
 #include "tb_common.hpp"
 
@@ -454,7 +455,14 @@ __device__ __forceinline__ void gemm_grouped_tile(const GemmLaunch& L, int g)
     }
 
     // ----------------------------------------------------------------- epilogue
-    const Out alpha = (Out)P.alpha, beta = (Out)P.beta;
+    Out alpha, beta;
+    if constexpr (sizeof(Out) == 4) {
+        alpha = P.alpha32;
+        beta = P.beta32;
+    } else {
+        alpha = P.alpha;
+        beta = P.beta;
+    }
     const bool lowerC = (P.modeC == kCLower);
     const long long ldc = P.ldc;
     const bool track = (P.eOut != nullptr);
@@ -466,23 +474,26 @@ __device__ __forceinline__ void gemm_grouped_tile(const GemmLaunch& L, int g)
     struct CIn {
         bool in;
         Out cold;
-        double cs;
+        Out cd;
     };
+    // the TRSM leaves' diagonal, in the tier's own precision
+    const Out* cdiv = static_cast<const Out*>(P.cdiv);
     auto fetch = [&](int i, int j) {
         CIn f;
         f.in = i < M && j < N && (!lowerC || i >= j);
-        f.cold = (Out)masked_load(Cb + (f.in ? (long long)i + (long long)j * ldc : 0ll),
-                                  f.in && beta != Out(0));
-        f.cs = P.cscale ? masked_load(P.cscale + (f.in ? j : 0), f.in) : 1.0;
+        f.cold = masked_load(Cb + (f.in ? (long long)i + (long long)j * ldc : 0ll),
+                             f.in && beta != Out(0));
+        f.cd = cdiv ? masked_load(cdiv + (f.in ? j : 0), f.in) : Out(1);
         return f;
     };
     auto put = [&](int i, int j, Out v, const CIn& f) {
         if (f.in) {
             Out* c = Cb + (long long)i + (long long)j * ldc;
             Out w;
-            if (P.cscale) {
-                const double cold = (beta == Out(0)) ? 0.0 : (double)f.cold;
-                w = (Out)(((double)alpha * (double)v + (double)beta * cold) * f.cs);
+            if (cdiv) {
+                const Out t = (beta == Out(0)) ? Out(0) : beta * f.cold;
+                if constexpr (sizeof(Out) == 4) w = div_sum_f32(alpha * v, t, f.cd);
+                else w = (alpha * v + t) / f.cd;
             } else {
                 w = (beta == Out(0)) ? alpha * v : alpha * v + beta * f.cold;
             }

@@ -118,13 +118,13 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     std::vector<TrtriJob> tj;
     // workspace offsets, fixed up to pointers after the allocation
     struct Off {
-        long long eRow = -1, eSnap = -1, eL = -1, eZ = -1, Z = -1, dinv = -1;
+        long long eRow = -1, eSnap = -1, eL = -1, eZ = -1, Z = -1, ldiag = -1;
         long long LpU = -1, LpA = -1, Zp = -1, Ap = -1, ldp = 0, esum = 0;   // planes: halves
         long long LpU1 = -1, LpA1 = -1;
         int kshift1 = 0;
     };
     std::vector<Off> off(count);
-    long long nexps = 0, nsnap = 0, nz = 0, ndinv = 0, nhalves = 0, maxk = 64;
+    long long nexps = 0, nsnap = 0, nz = 0, ndiag = 0, nhalves = 0, maxk = 64;
     auto rup = [](long long v, long long q) { return (v + q - 1) / q * q; };
     auto reserve_halves = [&](long long h) {
         const long long o = nhalves;
@@ -217,8 +217,8 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         nsnap += e.m;
         off[i].Z = nz;
         nz += z;
-        off[i].dinv = ndinv;
-        ndinv += e.n;
+        off[i].ldiag = ndiag;
+        ndiag += e.n;
         // L's rows in planes: 32 columns of slack past n (and past the shift), so that a
         // 32-k stage starting inside the row never leaves it
         off[i].kshift1 = resid > 0 ? 8 - resid : 0;
@@ -249,7 +249,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     const size_t oExp = ar.reserve(sizeof(int) * std::max<long long>(1, nexps));
     const size_t oSnap = ar.reserve(sizeof(int) * std::max<long long>(1, nsnap));
     const size_t oZ = ar.reserve(sizeof(float) * std::max<long long>(1, nz));
-    const size_t oD = ar.reserve(sizeof(double) * std::max<long long>(1, ndinv));
+    const size_t oD = ar.reserve(sizeof(float) * std::max<long long>(1, ndiag));
     // job tables are sized after the pointers are known; reserve their upper bounds
     const size_t oRj0 = ar.reserve(sizeof(RowexpJob) * (2 * (size_t)count + 1));
     const size_t oRj1 = ar.reserve(sizeof(RowexpJob) * (size_t)std::max<size_t>(1, steps.size() + 1));
@@ -276,7 +276,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     b->nexps = nexps;
     int* snap = reinterpret_cast<int*>(d + oSnap);
     float* Z = reinterpret_cast<float*>(d + oZ);
-    double* dinv = reinterpret_cast<double*>(d + oD);
+    float* ldiag = reinterpret_cast<float*>(d + oD);
     b->rj[0] = reinterpret_cast<RowexpJob*>(d + oRj0);
     b->rj[1] = reinterpret_cast<RowexpJob*>(d + oRj1);
     b->tj = reinterpret_cast<TrtriJob*>(d + oTj);
@@ -346,7 +346,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         I.eL = b->exps + o.eL;
         I.eZ = b->exps + o.eZ;
         I.Z = Z + o.Z;
-        I.dinv = dinv + o.dinv;
+        I.ldiag = ldiag + o.ldiag;
         // exponents of B's rows and L's rows (split updates), as trsm_run scans them
         add_rowexp(rj[0], b->rjblocks[0], e.C, e.ldc, kOpN, e.m, e.n, I.eRow);
         add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpTriL, e.n, e.n, (int*)I.eL);
@@ -389,7 +389,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
             t.s = (int)s;
             t.count = (int)(r - q);
             t.Z = Z + o.Z + z;
-            t.dinv = dinv + o.dinv + lo;
+            t.ldiag = ldiag + o.ldiag + lo;
             t.colBlocks = (int)((s + kTrtriCols - 1) / kTrtriCols);
             t.blockBegin = b->tjblocks;
             b->tjblocks += t.colBlocks * t.count;

@@ -22,8 +22,9 @@
 // nb with the update B2 -= X1*A21^T between the halves, a panel update between
 // chunks.  Leaves (the reference's cublasStrsm) are solved as X := B * inv(L_jj)^T,
 // written as (B + B W^T) D^-1 with L_jj = (I + N) D and W = (I + N)^-1 - I: all the
-// W are formed up front in FP64 by one batched kernel, and each leaf is one in-place
-// 64 x 256-tile GEMM at FP32 accuracy (split MFMA) with D^-1 applied in its epilogue.
+// W are formed up front by one batched kernel (in FP32 for the FP32-class tiers, FP64
+// for the FP64 tiers), and each leaf is one in-place 64 x 256-tile GEMM at FP32
+// accuracy (split MFMA) whose epilogue divides by the diagonal.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -183,6 +184,8 @@ GemmProb make_prob(long m, long n, long k, const void* A, long lda, int modeA, c
     p.ldc = ldc;
     p.alpha = alpha;
     p.beta = beta;
+    p.alpha32 = (float)alpha;
+    p.beta32 = (float)beta;
     p.m = (int)m;
     p.n = (int)n;
     p.k = (int)k;
@@ -524,7 +527,7 @@ struct TrsmCtx {
     int* eZ;         // exponents of the rows of each W_jj (columns of Z)
     int* eX;         // the eBX values the planes in xmem were scaled with
     char* xmem;      // FP16 planes of the solved block read by the current update / leaf
-    double* dinv;    // 1 / L(j, j) for every column j
+    char* ldiag;     // L(j, j) for every column j (element size es)
     std::vector<Leaf> leaves;
     std::vector<PlanOp> ops;    // the schedule, recorded by the collecting pass
     std::vector<size_t> zoff;   // element offset of each leaf's Z block (= s * first row)
@@ -597,8 +600,9 @@ struct TrsmCtx {
 
     // leaf: B(:, off:off+s) := B * inv(L_jj)^T = (B + B W^T) D^-1, s <= kLeafMax.  The
     // GEMM only forms the small correction B W^T (|W| ~ |offdiag L| / |diag L|) on top
-    // of B, and the diagonal scaling is one FP64 step in the epilogue: the rounding of
-    // the dominant diagonal term happens once, as in substitution.
+    // of B, and the epilogue divides B + B W^T by the diagonal in the tier's precision
+    // (FP32: TwoSum-corrected, about one rounding): the rounding of the dominant
+    // diagonal term happens once, as in substitution.
     int apply_leaf(long off, long s)
     {
         if (collect) {
@@ -609,7 +613,7 @@ struct TrsmCtx {
         const size_t q = next_leaf++;
         GemmProb p = make_prob(m, s, s, Bcol(off), ldb, kOpN, Z + zoff[q] * es, s, kOpN, Bcol(off),
                                ldb, 1.0, 1.0, kCFull);
-        p.cscale = dinv + off;   // X = (B + B W^T) D^-1
+        p.cdiv = ldiag + off * es;   // X = (B + B W^T) D^-1
         if (acc_tier == kSplitF16) {
             p.B = Zp.p + eoff[q] * Zp.ldp;
             p.ldb = Zp.ldp;
@@ -617,7 +621,12 @@ struct TrsmCtx {
             p.srcB = kSrcPlanes;
             p.eB = eZ + eoff[q];
             p.eOut = eBX;
-            TB_TRY(x_planes(acc_tier, off, s, &p));
+            // op(A) = the leaf's own block of B, read in FP32 and split on its way to
+            // LDS (no separate plane conversion): a leaf tile owns all s columns of its
+            // rows, so each element is converted once, and no other tile of the launch
+            // raises those rows' eBX while the tile reads it
+            p.srcA = kSrcF32;
+            p.eA = eBX;
         }
         GemmQueue g(h, acc_tier, kShapeLeaf, p.srcA, p.srcB);
         TB_TRY(g.add(p));
@@ -717,7 +726,7 @@ TrsmLayout trsm_plan(TrsmCtx& c)
     Arena a;
     t.oZ = a.reserve(z * c.es);
     t.oE = a.reserve(sizeof(int) * (2 * c.m + c.n + e));
-    t.oD = a.reserve(sizeof(double) * c.n);
+    t.oD = a.reserve(c.es * c.n);
     t.oZp = c.acc_tier == kSplitF16 ? a.reserve(planes_bytes(planes_of(c.h, kSplitF16), (long)e, kLeafMax)) : 0;
     t.oLp = c.l_planes() ? a.reserve(planes_bytes(planes_of(c.h, c.upd_tier), c.n, c.n, c.kshift)) : 0;
     // planes of the widest solved block an update or a leaf reads
@@ -737,9 +746,8 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
     c.eZ = c.eL + c.n;
     c.eX = c.eZ + t.esum;
     c.xmem = ws + t.oXp;
-    c.dinv = reinterpret_cast<double*>(ws + t.oD);
+    c.ldiag = ws + t.oD;
     hipStream_t st = c.h->stream;
-    const bool acc_f64 = tier_is_f64(c.acc_tier);
     if (c.split_exps()) {
         TB_HIP(fill_i32(c.eBX, c.m + c.n + (long long)t.esum, kExpZero, st));
         TB_HIP(rowexp(false, c.B, c.ldb, kOpN, (int)c.m, (int)c.n, 1, 0, c.eBX, 0, st));
@@ -766,8 +774,8 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
         const int cnt = (int)(r - q);
         hipEvent_t e0;
         TB_TRY(prof_begin(c.h, &e0));
-        TB_HIP(trtri_lower_t(c.f64, acc_f64, c.Aat(off, off), c.lda, s * (c.lda + 1), (int)s, cnt,
-                             c.Z + c.zoff[q] * c.es, c.dinv + off, st));
+        TB_HIP(trtri_lower_t(c.f64, c.Aat(off, off), c.lda, s * (c.lda + 1), (int)s, cnt,
+                             c.Z + c.zoff[q] * c.es, c.ldiag + off * c.es, st));
         TB_TRY(prof_end(c.h, kProfTrtri, e0, (double)s * s * s / 3.0 * cnt));
         if (c.acc_tier == kSplitF16) {
             // the blocks of the run stacked: row R = q'*s + j at Z + R*s, K = s

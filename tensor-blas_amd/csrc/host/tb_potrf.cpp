@@ -1,7 +1,8 @@
 // Recursive Cholesky factorization A = L L^T (lower), composed from the library's own
 // SYRK and TRSM: the downstream caller these BLAS-3 routines exist for.
 //
-//   potrf(A):  n <= 128: factor the diagonal block on the device in FP64 (potrf.hip)
+//   potrf(A):  n <= 128: factor the diagonal block on the device in the matrix's
+//              precision, FP32 or FP64 (potrf.hip)
 //              else, with A = [A11 . ; A21 A22] split at n1 (a multiple of 64 near n/2):
 //                potrf(A11)
 //                A21 := A21 L11^-T          tb_trsm  (X L^T = B, right/lower/T)

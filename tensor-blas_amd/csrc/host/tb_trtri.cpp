@@ -1,8 +1,9 @@
 // Triangular inverse in place, A := inv(A) (LAPACK dtrtri's contract), composed from the
 // library's TRSM and the diagonal-block inversion kernel of the TRSM leaves.
 //
-//   lower, order s:  s <= 256: invert the diagonal block on the device in FP64
-//                    (trtri_lower_t, then trtri_store writes D^-1 (I + W))
+//   lower, order s:  s <= 256: invert the diagonal block on the device in the matrix's
+//                    precision (FP32 for the FP32-class tiers; trtri_lower_t, then
+//                    trtri_store writes D^-1 (I + W))
 //                    else, with A = [L11 . ; L21 L22] split at n1 (a multiple of 64 near
 //                    s/2), inv(A) = [X11 . ; X21 X22] with X21 = -X22 L21 X11:
 //                      X11, X22 recursively, in place
@@ -44,7 +45,7 @@ struct TrtriCtx {
     int leaf(long off, long s)
     {
         const bool unit = diag == 'U';
-        const size_t zb = (size_t)s * s * 8, db = (size_t)s * 8;
+        const size_t zb = (size_t)s * s * es, db = (size_t)s * es;
         const size_t ob = unit ? ((size_t)s * s * es + 255) / 256 * 256 : 0;
         char* ws = nullptr;
         TB_TRY(workspace_get(h, ob + zb + db + 256, &ws, 2));
@@ -55,12 +56,13 @@ struct TrtriCtx {
             src = ws;
             ld = s;
         }
-        double* Z = reinterpret_cast<double*>(ws + ob);
-        double* dinv = reinterpret_cast<double*>(ws + ob + zb);
+        // in the matrix's own precision: FP32 blocks for the FP32-class tiers
+        char* Z = ws + ob;
+        char* ldiag = ws + ob + zb;
         hipEvent_t e0;
         TB_TRY(prof_begin(h, &e0));
-        TB_HIP(trtri_lower_t(f64(), true, src, ld, 0, (int)s, 1, Z, dinv, h->stream));
-        TB_HIP(trtri_store(f64(), Z, dinv, (int)s, at(off, off), lda, unit, h->stream));
+        TB_HIP(trtri_lower_t(f64(), src, ld, 0, (int)s, 1, Z, ldiag, h->stream));
+        TB_HIP(trtri_store(f64(), Z, ldiag, (int)s, at(off, off), lda, unit, h->stream));
         return prof_end(h, kProfTrtri, e0, (double)s * s * s / 3.0);
     }
 

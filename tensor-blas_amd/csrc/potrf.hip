@@ -1,7 +1,8 @@
 // Diagonal-block Cholesky of the recursive POTRF (host/tb_potrf.cpp).
 //
 // One workgroup factors one s x s diagonal block (s <= kPotrfLeaf = 128) in place in
-// FP64, A_jj = L_jj L_jj^T, lower triangle only, right-looking with the block in
+// the arithmetic of the matrix (FP32 for the FP32-class tiers, FP64 for the FP64
+// tiers), A_jj = L_jj L_jj^T, lower triangle only, right-looking with the block in
 // registers.  Thread tiles are 4 x 4; a wave holds an 8 x 8 block of tiles (32 x 32
 // elements), and the ten waves hold the ten lower-triangular such blocks, so a wave
 // whose rows or columns are all finished skips a step as a whole.
@@ -28,12 +29,15 @@ namespace {
 static_assert(kPotrfLeaf == 128, "4 x 4 blocks of 8 x 8 tiles of 4 x 4");
 constexpr int kLeafWaves = 10;                   // lower-triangular 32 x 32 blocks
 
+__device__ __forceinline__ float leaf_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double leaf_sqrt(double x) { return __builtin_sqrt(x); }
+
 template <typename T>
 __global__ void __launch_bounds__(64 * kLeafWaves)
 potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* __restrict__ info)
 {
-    __shared__ double colA[2][kPotrfLeaf];        // column j of the current step
-    __shared__ double colB[2][kPotrfLeaf];        // column j + 1
+    __shared__ T colA[2][kPotrfLeaf];             // column j of the current step
+    __shared__ T colB[2][kPotrfLeaf];             // column j + 1
     const int tid = threadIdx.x;
     if (*info != 0) return;                       // an earlier block failed
     // wave w -> lower block (bi, bj): w = bi (bi + 1) / 2 + bj
@@ -42,7 +46,7 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
     const int bj = w - bi * (bi + 1) / 2;
     const int i0 = 32 * bi + 4 * (lane >> 3), k0 = 32 * bj + 4 * (lane & 7);
     const bool live = k0 <= i0 + 3 && i0 < s && k0 < s;   // tile touches the lower block
-    double a[4][4];
+    T a[4][4];
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii) {
         const int i = i0 + ii, ic = min(i, s - 1);
@@ -62,24 +66,24 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
     for (int j = 0; j < s; j += 2) {
         __syncthreads();
         const int b = (j >> 1) & 1;
-        const double* ca = colA[b];
-        const double* cb = colB[b];
+        const T* ca = colA[b];
+        const T* cb = colB[b];
         const bool has1 = j + 1 < s;              // uniform
-        const double piv0 = ca[j];
-        if (!(piv0 > 0.0)) {                      // uniform: every thread reads the same pivot
+        const T piv0 = ca[j];
+        if (!(piv0 > T(0))) {                      // uniform: every thread reads the same pivot
             if (tid == 0) atomicCAS(info, 0, (int)(off + j + 1));
             break;
         }
-        const double d0 = __builtin_sqrt(piv0);
-        const double r0 = 1.0 / d0;
-        const double l1 = has1 ? ca[j + 1] * r0 : 0.0;
-        const double piv1 = has1 ? cb[j + 1] - l1 * l1 : 1.0;
-        const bool ok1 = piv1 > 0.0;              // uniform; false for NaN
-        const double d1 = __builtin_sqrt(ok1 ? piv1 : 1.0);
-        const double r1 = 1.0 / d1;
+        const T d0 = leaf_sqrt(piv0);
+        const T r0 = T(1) / d0;
+        const T l1 = has1 ? ca[j + 1] * r0 : T(0);
+        const T piv1 = has1 ? cb[j + 1] - l1 * l1 : T(1);
+        const bool ok1 = piv1 > T(0);              // uniform; false for NaN
+        const T d1 = leaf_sqrt(ok1 ? piv1 : T(1));
+        const T r1 = T(1) / d1;
         const bool two = has1 && ok1;             // columns j and j+1 this step
         if (live && i0 + 3 >= j && k0 + 3 >= j) {   // rows or columns left to finish
-            double li0[4], li1[4], lk0[4], lk1[4];
+            T li0[4], li1[4], lk0[4], lk1[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 li0[q] = ca[i0 + q] * r0;
@@ -93,7 +97,7 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     const int k = k0 + kk;
-                    double v = a[ii][kk];
+                    T v = a[ii][kk];
                     if (k == j) {
                         v = i == j ? d0 : (i > j ? li0[ii] : v);
                     } else if (two && k == j + 1) {
@@ -108,8 +112,8 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
             // the tiles holding columns j+2, j+3 publish them (static indices only: a
             // select on a run-time column index would put the tile in scratch memory)
             if (two) {
-                double* na = colA[b ^ 1];
-                double* nb = colB[b ^ 1];
+                T* na = colA[b ^ 1];
+                T* nb = colB[b ^ 1];
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     if (k0 + kk == j + 2) {
@@ -136,7 +140,7 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int k = k0 + kk;
-            if (live && i < s && k <= i) A[i + (long long)k * lda] = (T)a[ii][kk];
+            if (live && i < s && k <= i) A[i + (long long)k * lda] = a[ii][kk];
         }
     }
 }

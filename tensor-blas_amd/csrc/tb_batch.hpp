@@ -33,7 +33,7 @@ struct BatchInst {
     const int* eL;               // TRSM: exponents of the rows of L
     const int* eZ;               // TRSM: exponents of the columns of each leaf's W^T, stacked
     const float* Z;              // TRSM: W^T = (D inv(L_jj) - I)^T of each leaf, s x s, stacked
-    const double* dinv;          // TRSM: 1 / L(j, j)
+    const float* ldiag;          // TRSM: L(j, j), the leaves' FP32 divisors
     int step0, nsteps;           // TRSM: its steps in the step table
     // FP16 planes (K-contiguous rows, plane stride sp) made by the prologue:
     const void* LpU;             // TRSM: rows of L in the update tier's planes (ldpL)
@@ -98,7 +98,7 @@ struct TrtriJob {
     long long lda, bstride;      // stride between consecutive blocks of the run
     int s, count;                // block size, blocks in the run
     float* Z;                    // count s x s W^T blocks
-    double* dinv;                // count*s
+    float* ldiag;                // count*s: L(j, j) of each block
     int blockBegin, colBlocks;   // colBlocks = ceil(s / 64)
 };
 

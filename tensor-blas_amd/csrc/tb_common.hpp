@@ -57,13 +57,16 @@ struct GemmProb {
     const int* eB;    // split tier: exponent of each column of op(B) (and op(B2))
     int* eOut;        // if set: atomicMax of the exponents of the values written to each
                       // row of C (keeps a running per-row bound for later split GEMMs)
-    const double* cscale;  // if set: C(i,j) = (alpha*AB + beta*C)(i,j) * cscale[j], the
-                           // product and scaling in FP64 with one final rounding
+    const void* cdiv;      // if set: C(i,j) = (alpha*AB + beta*C)(i,j) / cdiv[j] (the TRSM
+                           // leaves' diagonal), in the tier's own precision: float* for the
+                           // FP32-class tiers (a TwoSum-corrected FP32 division), double*
+                           // for the FP64 tier
     long long lda, ldb, ldc;
     long long sA, sB, sC;      // batch strides (elements)
     long long seA, seB;        // batch strides of eA / eB
     long long spA, spB;        // plane strides of pre-converted FP16 operands (elements)
     double alpha, beta;
+    float alpha32, beta32;     // the same, rounded to FP32 on the host (FP32-class tiers)
     int m, n, k, batch;
     int modeA, modeB, modeC;
     int srcA, srcB;            // FP16 tiers: 0 = FP32 matrix, 1 = K-contiguous FP16 planes

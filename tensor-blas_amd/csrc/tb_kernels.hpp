@@ -68,17 +68,17 @@ hipError_t fill_i32(int* p, long long n, int v, hipStream_t s);
 // For q < nblocks: L_q = the s x s lower triangle at L + q*bstride (ld lda), factored
 // as L_q = (I + N) D with D = diag(L_q).  Writes Z_q = W^T with W = (I+N)^-1 - I =
 // D inv(L_q) - I (strictly lower; Z_q dense s x s, ld s, at Z + q*s*s) and
-// dinv[q*s + r] = 1 / L_q(r, r).  Then X L_q^T = B  <=>  X = (B + B W^T) D^-1.
-// FP64 arithmetic; Z is FP32 unless out_f64.  s <= 256.
+// ldiag[q*s + r] = L_q(r, r).  Then X L_q^T = B  <=>  X = (B + B W^T) D^-1.
+// Arithmetic, L, Z and ldiag in FP64 when f64, else FP32 (the FP32-class tiers keep
+// their leaves free of FP64).  s <= 256.
 // columns of inv(L) per 64-lane block (one wave) of the diagonal-block inversion
 constexpr int kTrtriCols = 8;
-hipError_t trtri_lower_t(bool in_f64, bool out_f64, const void* L, long long lda,
-                         long long bstride, int s, int nblocks, void* Z, double* dinv,
-                         hipStream_t st);
+hipError_t trtri_lower_t(bool f64, const void* L, long long lda, long long bstride, int s,
+                         int nblocks, void* Z, void* ldiag, hipStream_t st);
 // Writes inv(L_q) = D^-1 (I + W) into the lower triangle of the s x s block A (ld lda)
-// from trtri_lower_t's FP64 outputs (one block: Z s x s, dinv s); unit: the diagonal of
-// A is left as it is.  The strict upper triangle is not written.
-hipError_t trtri_store(bool f64, const double* Z, const double* dinv, int s, void* A, long long lda,
+// from trtri_lower_t's outputs of one block (Z s x s, ldiag s), in the same precision;
+// unit: the diagonal of A is left as it is.  The strict upper triangle is not written.
+hipError_t trtri_store(bool f64, const void* Z, const void* ldiag, int s, void* A, long long lda,
                        bool unit, hipStream_t st);
 // *info = min(*info, i + 1) over the i < n with A(i, i) == 0
 hipError_t diag_zero_scan(bool f64, const void* A, long long lda, long long n, int* info,
@@ -146,9 +146,10 @@ struct OzakiProb {
 };
 hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);
 
-// Cholesky of one s x s diagonal block (s <= kPotrfLeaf) in place, FP64 arithmetic,
-// lower triangle; A is float (f64 = false) or double.  A failing pivot at local column
-// j sets *info = off + j + 1 if *info is 0; a block with *info != 0 does nothing.
+// Cholesky of one s x s diagonal block (s <= kPotrfLeaf) in place, lower triangle, in
+// the arithmetic of A: FP32 (f64 = false, the FP32-class tiers) or FP64.  A failing
+// pivot at local column j sets *info = off + j + 1 if *info is 0; a block with
+// *info != 0 does nothing.
 constexpr int kPotrfLeaf = 128;
 hipError_t potrf_leaf(bool f64, void* A, long long lda, int s, long long off, int* info,
                       hipStream_t st);
