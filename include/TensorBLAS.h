@@ -216,40 +216,57 @@ TB_API int tb_gemm(tb_handle_t h, int tier, int transa, int transb, long m, long
             const void* A, long lda, const void* B, long ldb, double beta, void* C, long ldc);
 
 /* ------------------------------------------------------------ batched plans
- * Many independent SYRK and TRSM problems of mixed sizes, in one FP16 tier (PLAIN or
- * SPLIT), planned once and then run as one HIP graph: the row exponents and the
- * diagonal-block inverses of every instance (one launch each), then one persistent
- * kernel whose workgroups take work items -- SYRK lower tiles, and 64-row TRSM panels
- * that walk their instance's whole recursion -- from a queue sorted by decreasing
- * work.  Each batched instance gives the same bits as its single tb_syrk / tb_trsm
- * call in the same tier with the same nb.  The plan keeps the pointers it was given;
- * the data behind them may change between runs (TRSM solves B in place on every run).
- * No reference counterpart: the reference runs one problem per call. */
+ * Many independent SYRK, TRMM and TRSM problems of mixed sizes, in one FP16 tier (PLAIN
+ * or SPLIT), planned once and then run as one HIP graph: the row exponents, the FP16
+ * planes and the diagonal-block inverses of every instance (one launch each), then one
+ * persistent kernel whose workgroups take work items -- SYRK lower tiles, TRMM tiles,
+ * and 64-row TRSM panels that walk their instance's whole recursion -- from a queue
+ * sorted by decreasing work.  Each batched instance gives the same bits as its single
+ * tb_syrk / tb_trmm / tb_trsm call in the same tier with the same nb.  The plan keeps the
+ * pointers it was given; the data behind them may change between runs (TRSM solves B in
+ * place on every run).  Tile choices (64 x 256 or 128 x 128 tiles per SYRK / TRMM
+ * instance, 4 or 8 waves per workgroup) come from the library's tuning table, measured on
+ * MI355X and keyed by (tier, op, size bucket), or are timed on the plan's own instances
+ * (TB_BATCH_AUTOTUNE); they never change results.
+ * No reference counterpart: the reference runs one problem per call (its batched TRMM
+ * levels, BLAS/tc_trmm.cu:4-27, are within one problem). */
 typedef struct tb_batch_s* tb_batch_t;
-enum { TB_OP_SYRK = 0, TB_OP_TRSM = 1 };
+enum { TB_OP_SYRK = 0, TB_OP_TRSM = 1, TB_OP_TRMM = 2 };
 enum {
-    TB_BATCH_AUTOTUNE = 1,   /* time both SYRK tile shapes (64x256, 128x128) on scratch
-                                copies of the outputs and keep the faster */
-    TB_BATCH_NO_GRAPH = 2    /* launch the kernels directly on each run (no graph) */
-};
+    TB_BATCH_AUTOTUNE = 1,      /* time the tile choices on scratch copies of the outputs
+                                   (per op and size bucket) instead of using the table */
+    TB_BATCH_NO_GRAPH = 2,      /* launch the kernels directly on each run (no graph) */
+    TB_BATCH_TILES_PANEL = 4,   /* every SYRK / TRMM instance on 64 x 256 tiles */
+    TB_BATCH_TILES_SQUARE = 8,  /* every SYRK / TRMM instance on 128 x 128 tiles */
+    TB_BATCH_WAVES_4 = 16,      /* four waves per workgroup */
+    TB_BATCH_WAVES_8 = 32       /* eight waves per workgroup (SPLIT with 2 planes only) */
+};   /* the TILES and WAVES flags override the table or the autotuner for that choice */
 typedef struct {
     int op;              /* TB_OP_SYRK: C := alpha*A*A^T + beta*C, A n x k, lower C n x n
-                            TB_OP_TRSM: X*A^T = B in place, A = L n x n lower, B m x n */
-    long m, n, k;        /* SYRK: n, k (m unused);  TRSM: m, n (k unused) */
-    double alpha, beta;  /* SYRK only */
-    const float* A;      /* SYRK: A;  TRSM: L */
+                            TB_OP_TRSM: X*A^T = B in place, A = L n x n lower, B m x n
+                            TB_OP_TRMM: C := alpha*A*B, A = L m x m lower (upper part not
+                                        read), B and C m x n (beta unused) */
+    long m, n, k;        /* SYRK: n, k (m unused);  TRSM, TRMM: m, n (k unused) */
+    double alpha, beta;  /* SYRK; TRMM alpha */
+    const float* A;      /* SYRK: A;  TRSM, TRMM: L */
     long lda;
-    float* C;            /* SYRK: C;  TRSM: B */
+    float* C;            /* SYRK, TRMM: C;  TRSM: B */
     long ldc;
+    const float* B;      /* TRMM: B (unused otherwise) */
+    long ldb;
 } tb_batch_entry;
 TB_API int tb_batch_create(tb_handle_t handle, int tier, const tb_batch_entry* entries, int count,
                     long nb, int flags, tb_batch_t* batch);
 /* one run of the plan on the handle's stream (asynchronous) */
 TB_API int tb_batch_run(tb_batch_t batch);
-/* out[0] SYRK tile shape in use (0: 64x256, 1: 128x128), out[1] work items, out[2]
- * instances, out[3] / out[4] autotuning time of shape 0 / 1 in microseconds (0 when
- * not tuned), out[5] device bytes of the plan */
+/* out[0] waves per workgroup, out[1] work items, out[2] instances, out[3] SYRK / TRMM
+ * instances on 128 x 128 tiles, out[4] autotuning time in microseconds (0: the table's
+ * choices), out[5] device bytes of the plan */
 TB_API int tb_batch_info(tb_batch_t batch, long* out, int n);
+/* the plan's tile choices, in the tuning table's layout: out[b] the SYRK tile shape of
+ * size bucket b (0: 64 x 256, 1: 128 x 128), out[6 + b] the TRMM one, out[12] waves per
+ * workgroup; buckets: n < 128, < 256, < 512, < 1024, < 2048, >= 2048 */
+TB_API int tb_batch_tiles(tb_batch_t batch, int* out, int n);
 TB_API int tb_batch_destroy(tb_batch_t batch);
 
 /* ------------------------------------------------------------- request loop

@@ -1,7 +1,9 @@
 // Persistent kernel of a batch plan (tb_batch.hpp) and its batched prologue kernels.
 //
 // One workgroup per CU loops: take the next work item from an atomic counter (items
-// are sorted by decreasing work), run it, until the list is exhausted.  A TRSM item
+// are sorted by decreasing work), run it, until the list is exhausted.  A SYRK or TRMM
+// item is one output tile, 64 x 256 or 128 x 128 as the plan chose for its instance,
+// on operands the prologue put into FP16 planes.  A TRSM item
 // walks its instance's recursion for 64 rows: at each step it converts its rows of the
 // solved block (the leaf's B block, or the update's X1) into FP16 planes in its own
 // scratch, with the exponents the single call uses; then each leaf X = (B + B W^T) D^-1
@@ -30,11 +32,12 @@ constexpr int sq_stages() { return NPL == 3 ? 3 : 4; }
 template <int NPL, int SRC>
 constexpr int panel_nst() { return SRC == kSrcPlanes ? panel_stages<NPL>() : 2; }
 
-// waves per workgroup: eight (two per SIMD, as the single calls' split tiles) when every
-// tile is 2-plane; 64-row panels of 1 or 3 planes do not split into whole direct-to-LDS
-// instructions over eight waves, so those plans keep four
+// waves per workgroup: four, or eight (two per SIMD, as the single calls' split tiles)
+// when every tile is 2-plane; 64-row panels of 1 or 3 planes do not split into whole
+// direct-to-LDS instructions over eight waves.  Both give the same bits: each element
+// sums the same MFMA chain in the same order.
 template <int UNPL, int ANPL>
-constexpr int batch_waves() { return UNPL == 2 && ANPL == 2 ? 8 : 4; }
+constexpr bool waves_ok(int nw) { return nw == 4 || (nw == 8 && UNPL == 2 && ANPL == 2); }
 
 template <int NPL, int SRC, int NW>
 using CfgPanel = Gemm16Cfg<NPL, SRC, SRC, kPanelRows, kPanelCols, 1, NW, panel_nst<NPL, SRC>()>;
@@ -219,14 +222,14 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
     }
 }
 
-template <int UNPL, int ANPL, int SQ>
-__global__ void __launch_bounds__((64 * batch_waves<UNPL, ANPL>()))
+template <int UNPL, int ANPL, int NW>
+__global__ void __launch_bounds__(64 * NW)
 batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* __restrict__ insts,
              const BatchStep* __restrict__ steps, int* counter, BatchScratch S)
 {
-    constexpr int NW = batch_waves<UNPL, ANPL>();
+    static_assert(waves_ok<UNPL, ANPL>(NW), "waves");
     constexpr size_t kPanels = cmax(sizeof(LdsPanel<UNPL, kPP, NW>), sizeof(LdsPanel<ANPL, kPP, NW>));
-    constexpr size_t kLds = cmax(cmax(kPanels, SQ ? sizeof(LdsSq<UNPL, NW>) : 0), 64 * 65 * sizeof(float));
+    constexpr size_t kLds = cmax(cmax(kPanels, sizeof(LdsSq<UNPL, NW>)), 64 * 65 * sizeof(float));
     __shared__ __attribute__((aligned(1024))) char raw[kLds];
     __shared__ int next_s;
     for (;;) {
@@ -240,13 +243,21 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
         if (I.op == kBatchTrsm) {
             trsm_panel<UNPL, ANPL, NW>(I, steps, w.i0, S, raw);
         } else {
+            // SYRK: lower tile of A A^T (one operand); TRMM: full tile of L B with L's rows
+            // zero above the diagonal (the tile's k range stops at its last row)
+            const bool trmm = I.op == kBatchTrmm;
             GemmProb P = {};
-            P.m = P.n = I.n;
-            P.k = I.k;
-            P.A = P.B = I.Ap;
-            P.lda = P.ldb = I.ldpA;
-            P.spA = P.spB = I.spA;
-            P.modeA = P.modeB = kOpN;
+            P.m = trmm ? I.m : I.n;
+            P.n = I.n;
+            P.k = trmm ? I.m : I.k;
+            P.A = I.Ap;
+            P.lda = I.ldpA;
+            P.spA = I.spA;
+            P.B = trmm ? I.Bp : I.Ap;
+            P.ldb = trmm ? I.ldpB : I.ldpA;
+            P.spB = trmm ? I.spB : I.spA;
+            P.modeA = trmm ? kOpTriL : kOpN;
+            P.modeB = kOpN;
             P.srcA = P.srcB = kPP;
             P.C = I.C;
             P.ldc = I.ldc;
@@ -254,10 +265,11 @@ batch_kernel(const BatchItem* __restrict__ items, int nitems, const BatchInst* _
             P.beta = I.beta;
             P.alpha32 = I.alpha;
             P.beta32 = I.beta;
-            P.modeC = kCLower;
-            P.eA = P.eB = I.eRow;
+            P.modeC = trmm ? kCFull : kCLower;
+            P.eA = I.eRow;
+            P.eB = trmm ? I.eCol : I.eRow;
             P.batch = 1;
-            if constexpr (SQ) {
+            if (w.shape == kShapeSq) {
                 auto* L = reinterpret_cast<LdsSq<UNPL, NW>*>(raw);
                 gemm16_tile<UNPL, kPP, kPP, 128, 128, 2, NW / 2, sq_stages<UNPL>()>(
                     P, 0, w.i0, w.j0, L->ring, L->scA, L->scB, L->emax);
@@ -316,18 +328,12 @@ batch_planes_kernel(const PlanesJob* __restrict__ jobs, int njobs)
         split_planes_block<3>(J.X, J.ld, J.mode, J.rows, J.K, J.e, nullptr, out, J.ldp, J.sp, J.kshift, bx, by, t);
 }
 
-template <int UNPL, int ANPL>
-hipError_t run_sq(int sq, const BatchItem* items, int nitems, const BatchInst* insts,
-                  const BatchStep* steps, int* counter, const BatchScratch& S, int grid,
-                  hipStream_t s)
+template <int UNPL, int ANPL, int NW>
+hipError_t run_nw(const BatchItem* items, int nitems, const BatchInst* insts, const BatchStep* steps,
+                  int* counter, const BatchScratch& S, int grid, hipStream_t s)
 {
-    const dim3 block(64 * batch_waves<UNPL, ANPL>());
-    if (sq)
-        hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 1>), dim3(grid), block, 0, s, items, nitems, insts,
-                           steps, counter, S);
-    else
-        hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, 0>), dim3(grid), block, 0, s, items, nitems, insts,
-                           steps, counter, S);
+    hipLaunchKernelGGL((batch_kernel<UNPL, ANPL, NW>), dim3(grid), dim3(64 * NW), 0, s, items, nitems,
+                       insts, steps, counter, S);
     return hipGetLastError();
 }
 
@@ -364,16 +370,19 @@ hipError_t batch_planes(const PlanesJob* jobs, int njobs, int nblocks, hipStream
     return hipGetLastError();
 }
 
-hipError_t batch_run(int unpl, int anpl, int sq, const BatchItem* items, int nitems,
+hipError_t batch_run(int unpl, int anpl, int waves, const BatchItem* items, int nitems,
                      const BatchInst* insts, const BatchStep* steps, int* counter,
                      const BatchScratch& S, int grid, hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
     if (grid <= 0) return hipErrorInvalidValue;
-    if (unpl == 1 && anpl == 2) return run_sq<1, 2>(sq, items, nitems, insts, steps, counter, S, grid, s);
-    if (unpl == 1 && anpl == 3) return run_sq<1, 3>(sq, items, nitems, insts, steps, counter, S, grid, s);
-    if (unpl == 2 && anpl == 2) return run_sq<2, 2>(sq, items, nitems, insts, steps, counter, S, grid, s);
-    if (unpl == 3 && anpl == 3) return run_sq<3, 3>(sq, items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 2 && anpl == 2 && waves == 8)
+        return run_nw<2, 2, 8>(items, nitems, insts, steps, counter, S, grid, s);
+    if (waves != 4) return hipErrorInvalidValue;
+    if (unpl == 1 && anpl == 2) return run_nw<1, 2, 4>(items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 1 && anpl == 3) return run_nw<1, 3, 4>(items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 2 && anpl == 2) return run_nw<2, 2, 4>(items, nitems, insts, steps, counter, S, grid, s);
+    if (unpl == 3 && anpl == 3) return run_nw<3, 3, 4>(items, nitems, insts, steps, counter, S, grid, s);
     return hipErrorInvalidValue;
 }
 

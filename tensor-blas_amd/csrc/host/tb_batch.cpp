@@ -1,11 +1,19 @@
-// Batch plans (tb_batch_*): many independent SYRK / TRSM instances of mixed sizes in one
-// HIP graph -- batched row exponents and diagonal-block inverses, then the persistent
-// batch kernel of batch.hip over a work list sorted by decreasing work.
+// Batch plans (tb_batch_*): many independent SYRK / TRMM / TRSM instances of mixed sizes
+// in one HIP graph -- batched row exponents, diagonal-block inverses and plane
+// conversions, then the persistent batch kernel of batch.hip over a work list sorted by
+// decreasing work.
 //
 // The plan reproduces the single calls exactly: the TRSM schedule comes from the same
-// planner (trsm_schedule, BLAS/tc_trsm.cu:31-137), the exponents from the same row
-// scans, the inverses from the same trtri body, and the tiles run the same MFMA
-// sequence per element.  tests/test_gpu_batch.py checks batch == single calls bitwise.
+// planner (trsm_schedule, BLAS/tc_trsm.cu:31-137; the reference's batched TRMM levels,
+// BLAS/tc_trmm.cu:4-27, become output tiles of one triangular product), the exponents
+// from the same row scans, the inverses from the same trtri body, and the tiles run the
+// same MFMA sequence per element.  tests/test_gpu_batch.py checks batch == single calls
+// bitwise.
+//
+// Tile choices: each SYRK / TRMM instance runs on 64 x 256 or 128 x 128 tiles, and the
+// plan's workgroups have 4 or 8 waves (2-plane plans).  They come from the tuning table
+// (batch_tuning.inc, measured on MI355X by python/tune_batch.py), keyed by (tier, op,
+// size bucket), or, with TB_BATCH_AUTOTUNE, are timed on the plan's own instances.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -16,14 +24,37 @@
 
 using namespace tb;
 
+namespace {
+
+constexpr int kBuckets = 6;            // size buckets: n < 128, < 256, < 512, < 1024, < 2048, more
+constexpr int kTunedOps = 2;           // table rows: SYRK, TRMM
+#include "../batch_tuning.inc"          // kTunedShape[tier][op][bucket], kTunedWaves[tier][bucket]
+
+int bucket_of(long n)
+{
+    int b = 0;
+    for (long v = 128; b < kBuckets - 1 && n >= v; v *= 2) ++b;
+    return b;
+}
+
+// the tile choices of one plan
+struct Choice {
+    int shape[kTunedOps][kBuckets];    // BatchShape of SYRK (0) / TRMM (1) instances per bucket
+    int waves;                         // waves per workgroup of the persistent kernel
+};
+
+}  // namespace
+
 struct tb_batch_s {
     tb_handle_s* h = nullptr;
-    int tier = 0, unpl = 0, anpl = 0, sq = 0, flags = 0;
-    int count = 0, ninst_trsm = 0, ninst_syrk = 0;
+    int tier = 0, unpl = 0, anpl = 0, waves = 4, flags = 0;
+    int count = 0, ninst_trsm = 0, ninst_syrk = 0, ninst_trmm = 0, ninst_sq = 0;
+    Choice choice = {};
+    bool tuned = false;
     char* dev = nullptr;
     size_t dev_bytes = 0;
-    BatchItem* items[2] = {nullptr, nullptr};
-    int nitems[2] = {0, 0};
+    BatchItem* items = nullptr;
+    int nitems = 0;
     BatchInst* insts = nullptr;
     BatchStep* steps = nullptr;
     RowexpJob* rj[2] = {nullptr, nullptr};
@@ -40,7 +71,7 @@ struct tb_batch_s {
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    double tune_us[2] = {0.0, 0.0};
+    double tune_us = 0.0;        // autotuning time (0: the table's choices)
     double flops = 0.0;          // algorithmic flops of one run
 };
 
@@ -62,7 +93,34 @@ int check_entry(const tb_batch_entry& e)
         if (e.m > 0 && e.n > 0 && (!e.A || !e.C)) return TB_ERR_INVALID;
         return TB_SUCCESS;
     }
+    if (e.op == TB_OP_TRMM) {
+        if (!dim_ok(e.m) || !dim_ok(e.n)) return TB_ERR_INVALID;
+        if (e.lda < std::max(1l, e.m) || e.ldb < std::max(1l, e.m) || e.ldc < std::max(1l, e.m))
+            return TB_ERR_INVALID;
+        if (e.m > 0 && e.n > 0 && (!e.A || !e.B || !e.C)) return TB_ERR_INVALID;
+        return TB_SUCCESS;
+    }
     return TB_ERR_INVALID;
+}
+
+// size bucket of an instance: SYRK n, TRMM m (the triangle), TRSM n
+int inst_bucket(const tb_batch_entry& e) { return bucket_of(e.op == TB_OP_TRMM ? e.m : e.n); }
+
+Choice table_choice(int tier, const tb_batch_entry* E, int count)
+{
+    Choice c;
+    const int t = tier == kSplitF16 ? 1 : 0;
+    for (int o = 0; o < kTunedOps; ++o)
+        for (int q = 0; q < kBuckets; ++q) c.shape[o][q] = kTunedShape[t][o][q];
+    // waves: the table's choice for the bucket that holds most of the plan's TRSM work
+    double work[kBuckets] = {};
+    for (int i = 0; i < count; ++i)
+        if (E[i].op == TB_OP_TRSM) work[inst_bucket(E[i])] += (double)E[i].m * E[i].n * E[i].n;
+    int best = kBuckets - 1;
+    for (int q = 0; q < kBuckets; ++q)
+        if (work[q] > work[best]) best = q;
+    c.waves = kTunedWaves[t][best];
+    return c;
 }
 
 struct Costed {
@@ -85,7 +143,7 @@ hipError_t enqueue(const tb_batch_s* b, hipStream_t s)
     if ((e = batch_trtri(b->tj, b->ntj, b->tjblocks, s)) != hipSuccess) return e;
     if ((e = batch_rowexp(b->rj[1], b->nrj[1], b->rjblocks[1], s)) != hipSuccess) return e;
     if ((e = batch_planes(b->pj, b->npj, b->pjblocks, s)) != hipSuccess) return e;
-    return batch_run(b->unpl, b->anpl, b->sq, b->items[b->sq], b->nitems[b->sq], b->insts, b->steps,
+    return batch_run(b->unpl, b->anpl, b->waves, b->items, b->nitems, b->insts, b->steps,
                      b->counter, b->scratch, b->grid, s);
 }
 
@@ -100,8 +158,10 @@ void release(tb_batch_s* b)
     delete b;
 }
 
-// Builds the device tables and workspace of a plan (no graph, no tuning).
-int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb, tb_batch_s* b)
+// Builds the device tables and workspace of a plan with the tile choices ch (no graph,
+// no tuning).
+int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb, const Choice& ch,
+          tb_batch_s* b)
 {
     const bool split = tier == kSplitF16;
     b->h = h;
@@ -110,10 +170,12 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     b->unpl = planes_of(h, tier);
     b->anpl = planes_of(h, kSplitF16);      // leaves run in the split tier (as single calls do)
     b->grid = batch_grid(h->device);
+    b->choice = ch;
+    b->waves = (b->unpl == 2 && b->anpl == 2 && ch.waves == 8) ? 8 : 4;
 
     std::vector<BatchInst> insts(count);
     std::vector<BatchStep> steps;
-    std::vector<Costed> items[2];
+    std::vector<Costed> items;
     std::vector<RowexpJob> rj[2];
     std::vector<TrtriJob> tj;
     // workspace offsets, fixed up to pointers after the allocation
@@ -121,6 +183,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         long long eRow = -1, eSnap = -1, eL = -1, eZ = -1, Z = -1, ldiag = -1;
         long long LpU = -1, LpA = -1, Zp = -1, Ap = -1, ldp = 0, esum = 0;   // planes: halves
         long long LpU1 = -1, LpA1 = -1;
+        long long Bp = -1, ldpB = 0, eCol = -1;     // TRMM: columns of B in planes
         int kshift1 = 0;
     };
     std::vector<Off> off(count);
@@ -138,7 +201,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         const tb_batch_entry& e = E[i];
         BatchInst& I = insts[i];
         std::memset(&I, 0, sizeof(I));
-        I.op = e.op == TB_OP_SYRK ? kBatchSyrk : kBatchTrsm;
+        I.op = e.op == TB_OP_SYRK ? kBatchSyrk : (e.op == TB_OP_TRMM ? kBatchTrmm : kBatchTrsm);
         I.m = (int)e.m;
         I.n = (int)e.n;
         I.k = (int)e.k;
@@ -148,26 +211,37 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         I.lda = e.lda;
         I.C = e.C;
         I.ldc = e.ldc;
-        b->flops += e.op == TB_OP_SYRK ? (double)e.n * e.n * e.k : (double)e.m * e.n * e.n;
-        if (e.op == TB_OP_SYRK) {
-            ++b->ninst_syrk;
-            if (e.n == 0) continue;
+        b->flops += e.op == TB_OP_SYRK ? (double)e.n * e.n * e.k
+                  : e.op == TB_OP_TRMM ? (double)e.m * e.m * e.n : (double)e.m * e.n * e.n;
+        if (e.op == TB_OP_SYRK || e.op == TB_OP_TRMM) {
+            const bool trmm = e.op == TB_OP_TRMM;
+            ++(trmm ? b->ninst_trmm : b->ninst_syrk);
+            const long rows = trmm ? e.m : e.n, cols = e.n, K = trmm ? e.m : e.k;
+            if (rows == 0 || cols == 0) continue;
             if (split) {
                 off[i].eRow = nexps;
-                nexps += e.n;
+                nexps += rows;
+                if (trmm) {
+                    off[i].eCol = nexps;
+                    nexps += cols;
+                }
             }
-            off[i].ldp = rup(std::max(1l, e.k), 64);
-            off[i].Ap = reserve_halves((long long)b->unpl * e.n * off[i].ldp);
-            // 128 x 128 lower tiles (variant 1) and 64 x 256 tiles meeting the lower part (0)
-            const long t1 = (e.n + 127) / 128;
-            for (long ti = 0; ti < t1; ++ti)
-                for (long tj = 0; tj <= ti; ++tj)
-                    items[1].push_back({{i, (int)(ti * 128), (int)(tj * 128), 0}, 128.0 * 128.0 * e.k});
-            const long t0 = (e.n + 63) / 64;
-            for (long ti = 0; ti < t0; ++ti) {
-                const long last = std::min(e.n - 1, ti * 64 + 63) / 256;
-                for (long tj = 0; tj <= last; ++tj)
-                    items[0].push_back({{i, (int)(ti * 64), (int)(tj * 256), 0}, 64.0 * 256.0 * e.k});
+            off[i].ldp = rup(std::max(1l, K), 64);
+            off[i].Ap = reserve_halves((long long)b->unpl * rows * off[i].ldp);
+            if (trmm) {
+                off[i].ldpB = off[i].ldp;
+                off[i].Bp = reserve_halves((long long)b->unpl * cols * off[i].ldp);
+            }
+            // tiles: 128 x 128 (kShapeSq) or 64 x 256 (kShapePanel), lower ones only for
+            // SYRK; a TRMM tile's k range ends at its last row
+            const int shape = ch.shape[trmm ? 1 : 0][bucket_of(rows)];
+            if (shape == kShapeSq) ++b->ninst_sq;
+            const long bm = shape == kShapeSq ? 128 : 64, bn = shape == kShapeSq ? 128 : 256;
+            for (long i0 = 0; i0 < rows; i0 += bm) {
+                const long kk = trmm ? std::min(K, i0 + bm) : K;
+                const long jend = trmm ? cols : std::min(cols, i0 + bm);
+                for (long j0 = 0; j0 < jend; j0 += bn)
+                    items.push_back({{i, (int)i0, (int)j0, shape}, (double)bm * bn * kk});
             }
             continue;
         }
@@ -233,9 +307,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         off[i].Zp = reserve_halves((long long)b->anpl * ez * kZpLd);
         for (long r0 = 0; r0 < e.m; r0 += kPanelRows) {
             const double rows = (double)std::min<long>(kPanelRows, e.m - r0);
-            const Costed c = {{i, (int)r0, 0, 0}, rows * (double)e.n * (double)e.n * 0.5};
-            items[0].push_back(c);
-            items[1].push_back(c);
+            items.push_back({{i, (int)r0, 0, 0}, rows * (double)e.n * (double)e.n * 0.5});
         }
     }
 
@@ -243,8 +315,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     Arena ar;
     const size_t oInst = ar.reserve(sizeof(BatchInst) * std::max(1, count));
     const size_t oStep = ar.reserve(sizeof(BatchStep) * std::max<size_t>(1, steps.size()));
-    const size_t oIt0 = ar.reserve(sizeof(BatchItem) * std::max<size_t>(1, items[0].size()));
-    const size_t oIt1 = ar.reserve(sizeof(BatchItem) * std::max<size_t>(1, items[1].size()));
+    const size_t oIt = ar.reserve(sizeof(BatchItem) * std::max<size_t>(1, items.size()));
     const size_t oCnt = ar.reserve(sizeof(int));
     const size_t oExp = ar.reserve(sizeof(int) * std::max<long long>(1, nexps));
     const size_t oSnap = ar.reserve(sizeof(int) * std::max<long long>(1, nsnap));
@@ -269,8 +340,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     char* d = b->dev;
     b->insts = reinterpret_cast<BatchInst*>(d + oInst);
     b->steps = reinterpret_cast<BatchStep*>(d + oStep);
-    b->items[0] = reinterpret_cast<BatchItem*>(d + oIt0);
-    b->items[1] = reinterpret_cast<BatchItem*>(d + oIt1);
+    b->items = reinterpret_cast<BatchItem*>(d + oIt);
     b->counter = reinterpret_cast<int*>(d + oCnt);
     b->exps = reinterpret_cast<int*>(d + oExp);
     b->nexps = nexps;
@@ -330,6 +400,27 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
         BatchInst& I = insts[i];
         const Off& o = off[i];
         if (o.eRow >= 0) I.eRow = b->exps + o.eRow;
+        if (e.op == TB_OP_TRMM) {
+            if (e.m == 0 || e.n == 0) continue;
+            // rows of L (lower, zero above the diagonal) and columns of B, exponents and
+            // planes as the single call's fp16_product makes them
+            if (o.eCol >= 0) I.eCol = b->exps + o.eCol;
+            if (o.eRow >= 0) {
+                add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpTriL, e.m, e.m, I.eRow);
+                add_rowexp(rj[0], b->rjblocks[0], e.B, e.ldb, kOpT, e.n, e.m, (int*)I.eCol);
+            }
+            I.Ap = planes + o.Ap;
+            I.ldpA = o.ldp;
+            I.spA = (long long)e.m * o.ldp;
+            add_planes(e.A, e.lda, kOpTriL, e.m, e.m, split ? I.eRow : nullptr, b->unpl,
+                       planes + o.Ap, o.ldp, I.spA);
+            I.Bp = planes + o.Bp;
+            I.ldpB = o.ldpB;
+            I.spB = (long long)e.n * o.ldpB;
+            add_planes(e.B, e.ldb, kOpT, e.n, e.m, split ? I.eCol : nullptr, b->unpl,
+                       planes + o.Bp, o.ldpB, I.spB);
+            continue;
+        }
         if (e.op == TB_OP_SYRK) {
             if (o.eRow >= 0) add_rowexp(rj[0], b->rjblocks[0], e.A, e.lda, kOpN, e.n, e.k, I.eRow);
             if (e.n == 0) continue;
@@ -415,14 +506,12 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     if (!pj.empty())
         TB_HIP(hipMemcpy(b->pj, pj.data(), sizeof(PlanesJob) * pj.size(), hipMemcpyHostToDevice));
 
-    for (int v = 0; v < 2; ++v) {
-        sort_items(items[v]);
-        std::vector<BatchItem> flat(items[v].size());
-        for (size_t q = 0; q < flat.size(); ++q) flat[q] = items[v][q].it;
-        b->nitems[v] = (int)flat.size();
-        if (!flat.empty())
-            TB_HIP(hipMemcpy(b->items[v], flat.data(), sizeof(BatchItem) * flat.size(), hipMemcpyHostToDevice));
-    }
+    sort_items(items);
+    std::vector<BatchItem> flat(items.size());
+    for (size_t q = 0; q < flat.size(); ++q) flat[q] = items[q].it;
+    b->nitems = (int)flat.size();
+    if (!flat.empty())
+        TB_HIP(hipMemcpy(b->items, flat.data(), sizeof(BatchItem) * flat.size(), hipMemcpyHostToDevice));
     if (count > 0) TB_HIP(hipMemcpy(b->insts, insts.data(), sizeof(BatchInst) * count, hipMemcpyHostToDevice));
     if (!steps.empty())
         TB_HIP(hipMemcpy(b->steps, steps.data(), sizeof(BatchStep) * steps.size(), hipMemcpyHostToDevice));
@@ -433,13 +522,47 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
     return TB_SUCCESS;
 }
 
-// Times both SYRK tile shapes on scratch copies of the outputs (the real outputs are
-// not touched) and keeps the faster one.
-int autotune(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb, tb_batch_s* b)
+// Run time of a plan over entries E (on its handle's stream, HIP events; the faster of
+// two runs after a warm-up), in microseconds, or a negative status.
+double time_plan(tb_handle_s* h, int tier, const std::vector<tb_batch_entry>& E, long nb, const Choice& ch)
+{
+    tb_batch_s* t = new (std::nothrow) tb_batch_s();
+    if (!t) return -TB_ERR_NOMEM;
+    int st = build(h, tier, E.data(), (int)E.size(), nb, ch, t);
+    double best = -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st == TB_SUCCESS && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = TB_ERR_DEVICE;
+    if (st == TB_SUCCESS && enqueue(t, h->stream) != hipSuccess) st = TB_ERR_DEVICE;
+    for (int r = 0; r < 2 && st == TB_SUCCESS; ++r) {
+        float ms = 0.0f;
+        if (hipEventRecord(e0, h->stream) != hipSuccess || enqueue(t, h->stream) != hipSuccess ||
+            hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            st = TB_ERR_DEVICE;
+        else if (best < 0.0 || 1e3 * ms < best)
+            best = 1e3 * ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    release(t);
+    return st == TB_SUCCESS ? best : -(double)st;
+}
+
+// Times the tile choices on the plan's own instances, with scratch copies of their
+// outputs (the caller's data is not touched): for each (op, size bucket) of SYRK and
+// TRMM instances, their sub-plan on 64 x 256 against 128 x 128 tiles; for 2-plane plans
+// with TRSM instances, the TRSM sub-plan on four against eight waves.  Choices without
+// instances keep the table's values.  The choices never change results.
+int autotune(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb, Choice* ch,
+             double* us)
 {
     std::vector<tb_batch_entry> T(E, E + count);
     std::vector<void*> scratch;
     int st = TB_SUCCESS;
+    hipEvent_t a0 = nullptr, a1 = nullptr;
+    if (hipEventCreate(&a0) != hipSuccess || hipEventCreate(&a1) != hipSuccess) st = TB_ERR_DEVICE;
+    if (st == TB_SUCCESS && hipEventRecord(a0, h->stream) != hipSuccess) st = TB_ERR_DEVICE;
     for (int i = 0; i < count && st == TB_SUCCESS; ++i) {
         // the output is rows x cols with leading dimension ldc: only (cols-1)*ldc + rows
         // elements of it are the caller's, so it is copied as a 2D region
@@ -458,25 +581,49 @@ int autotune(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long 
             st = TB_ERR_DEVICE;
         T[i].C = static_cast<float*>(p);
     }
-    tb_batch_s* t = new (std::nothrow) tb_batch_s();
-    if (!t) st = TB_ERR_NOMEM;
-    if (st == TB_SUCCESS) st = build(h, tier, T.data(), count, nb, t);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (st == TB_SUCCESS && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
-        st = TB_ERR_DEVICE;
-    for (int v = 0; v < 2 && st == TB_SUCCESS; ++v) {
-        t->sq = v;
-        float ms = 0.0f;
-        if (enqueue(t, h->stream) != hipSuccess || hipEventRecord(e0, h->stream) != hipSuccess ||
-            enqueue(t, h->stream) != hipSuccess || hipEventRecord(e1, h->stream) != hipSuccess ||
-            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-            st = TB_ERR_DEVICE;
-        b->tune_us[v] = 1e3 * ms;
+    // SYRK / TRMM tile shapes per (op, bucket)
+    for (int o = 0; o < kTunedOps && st == TB_SUCCESS; ++o) {
+        const int op = o == 0 ? TB_OP_SYRK : TB_OP_TRMM;
+        for (int q = 0; q < kBuckets && st == TB_SUCCESS; ++q) {
+            std::vector<tb_batch_entry> sub;
+            for (const tb_batch_entry& e : T)
+                if (e.op == op && inst_bucket(e) == q) sub.push_back(e);
+            if (sub.empty()) continue;
+            double t[2];
+            for (int v = 0; v < 2 && st == TB_SUCCESS; ++v) {
+                Choice c = *ch;
+                c.shape[o][q] = v;
+                t[v] = time_plan(h, tier, sub, nb, c);
+                if (t[v] < 0.0) st = (int)-t[v];
+            }
+            if (st == TB_SUCCESS) ch->shape[o][q] = t[1] < t[0] ? kShapeSq : kShapePanel;
+        }
     }
-    if (st == TB_SUCCESS) b->sq = b->tune_us[1] < b->tune_us[0] ? 1 : 0;
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (t) release(t);
+    // waves per workgroup (2-plane plans): on the TRSM instances
+    if (st == TB_SUCCESS && planes_of(h, tier) == 2) {
+        std::vector<tb_batch_entry> sub;
+        for (const tb_batch_entry& e : T)
+            if (e.op == TB_OP_TRSM) sub.push_back(e);
+        if (!sub.empty()) {
+            double t[2];
+            for (int v = 0; v < 2 && st == TB_SUCCESS; ++v) {
+                Choice c = *ch;
+                c.waves = v == 0 ? 4 : 8;
+                t[v] = time_plan(h, tier, sub, nb, c);
+                if (t[v] < 0.0) st = (int)-t[v];
+            }
+            if (st == TB_SUCCESS) ch->waves = t[1] < t[0] ? 8 : 4;
+        }
+    }
+    float ms = 0.0f;
+    if (st == TB_SUCCESS && (hipEventRecord(a1, h->stream) != hipSuccess ||
+                             hipEventSynchronize(a1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, a0, a1) != hipSuccess))
+        st = TB_ERR_DEVICE;
+    *us = 1e3 * ms;
+    if (a0) (void)hipEventDestroy(a0);
+    if (a1) (void)hipEventDestroy(a1);
+    (void)hipStreamSynchronize(h->stream);
     for (void* p : scratch) (void)hipFree(p);
     return st;
 }
@@ -488,15 +635,30 @@ int tb_batch_create(tb_handle_t h, int tier, const tb_batch_entry* entries, int 
 {
     if (!h || !batch || count < 0 || (count > 0 && !entries) || nb <= 0) return TB_ERR_INVALID;
     if (tier != TB_TIER_PLAIN && tier != TB_TIER_SPLIT) return TB_ERR_INVALID;
-    if (flags & ~(TB_BATCH_AUTOTUNE | TB_BATCH_NO_GRAPH)) return TB_ERR_INVALID;
+    if (flags & ~(TB_BATCH_AUTOTUNE | TB_BATCH_NO_GRAPH | TB_BATCH_TILES_PANEL | TB_BATCH_TILES_SQUARE |
+                  TB_BATCH_WAVES_4 | TB_BATCH_WAVES_8))
+        return TB_ERR_INVALID;
+    if ((flags & TB_BATCH_TILES_PANEL) && (flags & TB_BATCH_TILES_SQUARE)) return TB_ERR_INVALID;
+    if ((flags & TB_BATCH_WAVES_4) && (flags & TB_BATCH_WAVES_8)) return TB_ERR_INVALID;
+    if ((flags & TB_BATCH_WAVES_8) && planes_of(h, tier) != 2) return TB_ERR_INVALID;
     for (int i = 0; i < count; ++i) TB_TRY(check_entry(entries[i]));
     *batch = nullptr;
     tb_batch_s* b = new (std::nothrow) tb_batch_s();
     if (!b) return TB_ERR_NOMEM;
     b->flags = flags;
-    int st = build(h, tier, entries, count, nb, b);
-    if (st == TB_SUCCESS && (flags & TB_BATCH_AUTOTUNE) && b->ninst_syrk > 0)
-        st = autotune(h, tier, entries, count, nb, b);
+    Choice ch = table_choice(tier, entries, count);
+    double tune_us = 0.0;
+    int st = TB_SUCCESS;
+    if (flags & TB_BATCH_AUTOTUNE) st = autotune(h, tier, entries, count, nb, &ch, &tune_us);
+    if (flags & (TB_BATCH_TILES_PANEL | TB_BATCH_TILES_SQUARE))
+        for (int o = 0; o < kTunedOps; ++o)
+            for (int q = 0; q < kBuckets; ++q)
+                ch.shape[o][q] = (flags & TB_BATCH_TILES_SQUARE) ? kShapeSq : kShapePanel;
+    if (flags & TB_BATCH_WAVES_4) ch.waves = 4;
+    if (flags & TB_BATCH_WAVES_8) ch.waves = 8;
+    if (st == TB_SUCCESS) st = build(h, tier, entries, count, nb, ch, b);
+    b->tune_us = tune_us;
+    b->tuned = (flags & TB_BATCH_AUTOTUNE) != 0;
     if (st == TB_SUCCESS && !(flags & TB_BATCH_NO_GRAPH)) {
         if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
             hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -531,9 +693,20 @@ int tb_batch_run(tb_batch_t b)
 int tb_batch_info(tb_batch_t b, long* out, int n)
 {
     if (!b || n < 0 || (n > 0 && !out)) return TB_ERR_INVALID;
-    const long v[6] = {b->sq, b->nitems[b->sq], b->count, (long)b->tune_us[0], (long)b->tune_us[1],
+    const long v[6] = {b->waves, b->nitems, b->count, b->ninst_sq, (long)b->tune_us,
                        (long)b->dev_bytes};
     for (int i = 0; i < n && i < 6; ++i) out[i] = v[i];
+    return TB_SUCCESS;
+}
+
+int tb_batch_tiles(tb_batch_t b, int* out, int n)
+{
+    if (!b || n < 0 || (n > 0 && !out)) return TB_ERR_INVALID;
+    int v[2 * kBuckets + 1];
+    for (int o = 0; o < kTunedOps; ++o)
+        for (int q = 0; q < kBuckets; ++q) v[o * kBuckets + q] = b->choice.shape[o][q];
+    v[2 * kBuckets] = b->waves;
+    for (int i = 0; i < n && i < 2 * kBuckets + 1; ++i) out[i] = v[i];
     return TB_SUCCESS;
 }
 

@@ -1,9 +1,11 @@
-// Device tables of a batch plan: many independent SYRK and TRSM instances of mixed
-// sizes run by one persistent kernel (batch.hip), planned on the host
+// Device tables of a batch plan: many independent SYRK, TRMM and TRSM instances of
+// mixed sizes run by one persistent kernel (batch.hip), planned on the host
 // (host/tb_batch.cpp).
 //
 // Work items:
 //   SYRK  one lower output tile C(i0:, j0:) of one instance;
+//   TRMM  one output tile C(i0:, j0:) of C = alpha L B (L lower, its strict upper part
+//         read as zero: the k range of a tile stops at its last row);
 //   TRSM  one panel of 64 rows of X*L^T = B of one instance.  Rows of X depend only
 //         on the same rows of B, so a panel walks the instance's whole recursion
 //         (BLAS/tc_trsm.cu:31-137: leaves and updates, in schedule order) by itself
@@ -15,20 +17,26 @@
 
 namespace tb {
 
-enum BatchOp : int { kBatchSyrk = 0, kBatchTrsm = 1 };
+enum BatchOp : int { kBatchSyrk = 0, kBatchTrsm = 1, kBatchTrmm = 2 };
+
+// tile shape of a SYRK / TRMM work item
+enum BatchShape : int { kShapePanel = 0,    // 64 x 256 (the TRSM panel tile)
+                        kShapeSq = 1 };     // 128 x 128
 
 constexpr int kPanelRows = 64;     // rows of a TRSM panel (= BM of the panel tiles)
 constexpr int kPanelCols = 256;    // columns of a panel tile (>= every leaf width)
 
 struct BatchInst {
     int op;
-    int m, n, k;                 // SYRK: n x k A; TRSM: m x n B, n x n L
-    float alpha, beta;           // SYRK
-    const float* A;              // SYRK: A; TRSM: L (lower part read)
+    int m, n, k;                 // SYRK: n x k A; TRSM: m x n B, n x n L; TRMM: m x m L, m x n B
+    float alpha, beta;           // SYRK, TRMM (beta 0)
+    const float* A;              // SYRK: A; TRSM, TRMM: L (lower part read)
     long long lda;
-    float* C;                    // SYRK: C (lower part written); TRSM: B, overwritten by X
+    float* C;                    // SYRK: C (lower part written); TRSM: B, overwritten by X;
+                                 // TRMM: C (m x n)
     long long ldc;
-    int* eRow;                   // split exponents: SYRK rows of A; TRSM running bound per row of B
+    int* eRow;                   // split exponents: SYRK rows of A; TRSM running bound per row
+                                 // of B; TRMM rows of L
     int* eSnap;                  // TRSM: eRow of a panel's rows as of the start of a step
     const int* eL;               // TRSM: exponents of the rows of L
     const int* eZ;               // TRSM: exponents of the columns of each leaf's W^T, stacked
@@ -44,8 +52,11 @@ struct BatchInst {
     int kshift1;
     const void* Zp;              // TRSM: rows of every leaf's W^T (leaf tier), ld kZpLd
     long long spZ;
-    const void* Ap;              // SYRK: rows of A
+    const void* Ap;              // SYRK: rows of A; TRMM: rows of L (zero above the diagonal)
     long long ldpA, spA;
+    const void* Bp;              // TRMM: columns of B (rows of B^T)
+    long long ldpB, spB;
+    const int* eCol;             // TRMM split exponents of B's columns
 };
 
 constexpr int kZpLd = 256;       // plane row length of the leaves' W^T (>= every leaf width)
@@ -67,8 +78,8 @@ struct BatchStep {
 
 struct BatchItem {
     int inst;
-    int i0, j0;                  // SYRK: tile origin; TRSM: first row of the panel (j0 = 0)
-    int pad;
+    int i0, j0;                  // SYRK / TRMM: tile origin; TRSM: first row of the panel (j0 = 0)
+    int shape;                   // SYRK / TRMM: BatchShape of the tile
 };
 
 // Batched prologue jobs: row exponents and diagonal-block inverses of all instances,

@@ -175,10 +175,11 @@ struct BatchScratch;
 hipError_t batch_rowexp(const RowexpJob* jobs, int njobs, int nblocks, hipStream_t s);
 hipError_t batch_trtri(const TrtriJob* jobs, int njobs, int nblocks, hipStream_t s);
 hipError_t batch_planes(const PlanesJob* jobs, int njobs, int nblocks, hipStream_t s);
-// the persistent batch kernel: unpl / anpl = FP16 planes of update (and SYRK) tiles /
-// leaf tiles, sq = SYRK tiles of 128 x 128 (1) or 64 x 256 (0); grid = workgroups
-// (each with its scratch in S); counter must hold 0
-hipError_t batch_run(int unpl, int anpl, int sq, const BatchItem* items, int nitems,
+// the persistent batch kernel: unpl / anpl = FP16 planes of update (and SYRK / TRMM)
+// tiles / leaf tiles, waves = 4, or 8 (2-plane plans only) per workgroup; each SYRK /
+// TRMM item names its own tile shape; grid = workgroups (each with its scratch in S);
+// counter must hold 0
+hipError_t batch_run(int unpl, int anpl, int waves, const BatchItem* items, int nitems,
                      const BatchInst* insts, const BatchStep* steps, int* counter,
                      const BatchScratch& S, int grid, hipStream_t s);
 // workgroups of the persistent batch kernel that one device runs at once

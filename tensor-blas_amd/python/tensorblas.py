@@ -73,6 +73,7 @@ def lib():
             "tb_batch_create": [_P, _I, _P, _I, _L, _I, _P],
             "tb_batch_run": [_P],
             "tb_batch_info": [_P, _P, _I],
+            "tb_batch_tiles": [_P, _P, _I],
             "tb_batch_destroy": [_P],
             "tb_serve": [_P, _I, _P, _P, _I, _P],
             "tb_reserve_workspace": [_P, _L, _L],
@@ -268,26 +269,31 @@ class BatchEntry(ctypes.Structure):
     _fields_ = [("op", ctypes.c_int), ("m", ctypes.c_long), ("n", ctypes.c_long),
                 ("k", ctypes.c_long), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
                 ("A", ctypes.c_void_p), ("lda", ctypes.c_long), ("C", ctypes.c_void_p),
-                ("ldc", ctypes.c_long)]
+                ("ldc", ctypes.c_long), ("B", ctypes.c_void_p), ("ldb", ctypes.c_long)]
 
 
-BATCH_OPS = {"syrk": 0, "trsm": 1}
+BATCH_OPS = {"syrk": 0, "trsm": 1, "trmm": 2}
 BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+BATCH_TILES = {None: 0, "panel": 4, "square": 8}
+BATCH_WAVES = {None: 0, 4: 16, 8: 32}
 
 
 class Batch:
-    """A batch plan: many SYRK / TRSM problems run as one HIP graph (tb_batch_*).
+    """A batch plan: many SYRK / TRMM / TRSM problems run as one HIP graph (tb_batch_*).
 
-    entries: dicts with op ('syrk' | 'trsm'), n, and m (trsm) or k (syrk), A, lda,
-    C, ldc (device addresses; for trsm A is L and C is B), alpha / beta (syrk)."""
+    entries: dicts with op ('syrk' | 'trmm' | 'trsm'), n, and m (trmm, trsm) or k (syrk),
+    A, lda, C, ldc, and B, ldb (trmm) as device addresses (for trsm A is L and C is B,
+    for trmm A is L), alpha / beta (syrk; trmm alpha)."""
 
-    def __init__(self, handle, tier, entries, nb=256, autotune=False, graph=True):
+    def __init__(self, handle, tier, entries, nb=256, autotune=False, graph=True, tiles=None,
+                 waves=None):
         arr = (BatchEntry * max(1, len(entries)))()
         for i, e in enumerate(entries):
             arr[i] = BatchEntry(BATCH_OPS[e["op"]], e.get("m", 0), e["n"], e.get("k", 0),
                                 e.get("alpha", 1.0), e.get("beta", 0.0), e["A"], e["lda"], e["C"],
-                                e["ldc"])
-        flags = (BATCH_AUTOTUNE if autotune else 0) | (0 if graph else BATCH_NO_GRAPH)
+                                e["ldc"], e.get("B", 0) or None, e.get("ldb", 0))
+        flags = ((BATCH_AUTOTUNE if autotune else 0) | (0 if graph else BATCH_NO_GRAPH) |
+                 BATCH_TILES[tiles] | BATCH_WAVES[waves])
         self._b = ctypes.c_void_p()
         self._handle = handle          # the plan runs on the handle's stream
         _check("tb_batch_create", lib().tb_batch_create(handle.ptr, tier_id(tier), arr, len(entries),
@@ -299,8 +305,15 @@ class Batch:
     def info(self):
         out = (ctypes.c_long * 6)()
         _check("tb_batch_info", lib().tb_batch_info(self._b, out, 6))
-        return {"syrk_tile": "128x128" if out[0] else "64x256", "items": out[1],
-                "instances": out[2], "tune_us": (out[3], out[4]), "device_bytes": out[5]}
+        return {"waves": out[0], "items": out[1], "instances": out[2], "square_tile_instances": out[3],
+                "tune_us": out[4], "device_bytes": out[5]}
+
+    def tiles(self):
+        """the plan's tile choices: {'syrk': [shape per bucket], 'trmm': [...], 'waves': w};
+        shape 0 = 64 x 256, 1 = 128 x 128; buckets n < 128, < 256, < 512, < 1024, < 2048, more"""
+        out = (ctypes.c_int * 13)()
+        _check("tb_batch_tiles", lib().tb_batch_tiles(self._b, out, 13))
+        return {"syrk": list(out[0:6]), "trmm": list(out[6:12]), "waves": out[12]}
 
     def close(self):
         if self._b:
