@@ -269,10 +269,10 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
                 if (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
             }
         }
-        _Float16* dst = out + (long long)r * ldp + c0 + cl0 + half * 8;
-        *reinterpret_cast<half8*>(dst) = hh;
-        if (NPL >= 2) *reinterpret_cast<half8*>(dst + sp) = mm;
-        if (NPL == 3) *reinterpret_cast<half8*>(dst + 2 * sp) = ll;
+        const int c = c0 + cl0 + half * 8;
+        *reinterpret_cast<half8*>(out + plane_off(NPL, ldp, sp, 0, r, c)) = hh;
+        if (NPL >= 2) *reinterpret_cast<half8*>(out + plane_off(NPL, ldp, sp, 1, r, c)) = mm;
+        if (NPL == 3) *reinterpret_cast<half8*>(out + plane_off(NPL, ldp, sp, 2, r, c)) = ll;
     }
 }
 

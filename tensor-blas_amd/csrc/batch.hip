@@ -65,7 +65,7 @@ __device__ __forceinline__ int load_exp(const int* p)
 }
 
 // Rows i0 .. i0+63 (those < m) of the column-major X (ld), columns 0 .. K-1, into NPL
-// planes at dst (plane p, row r, column k at dst[p*sp + r*ldp + k]) with columns
+// planes at dst (plane p, row r, column k at dst[plane_off(NPL, ldp, sp, p, r, k)]) with columns
 // K .. round_up(K, 64) and rows >= m zero.  Split planes are scaled per row by
 // 2^(15 - e[row]) exactly as split_planes_block converts them.  t: a 64 x 65 LDS tile.
 // NT threads (256 or 512): four per row, each converting 16 (NT = 256) or 8 columns of
@@ -122,10 +122,10 @@ __device__ __forceinline__ void panel_planes(const float* __restrict__ X, long l
                     if constexpr (NPL == 3) ll[j] = (_Float16)((r1 - (float)mq) * 2048.0f);
                 }
             }
-            _Float16* d = dst + (long long)rl * ldp + c0 + cl0 + half * 8;
-            *reinterpret_cast<half8*>(d) = hh;
-            if constexpr (NPL >= 2) *reinterpret_cast<half8*>(d + sp) = mm;
-            if constexpr (NPL == 3) *reinterpret_cast<half8*>(d + 2 * sp) = ll;
+            const int c = c0 + cl0 + half * 8;
+            *reinterpret_cast<half8*>(dst + plane_off(NPL, ldp, sp, 0, rl, c)) = hh;
+            if constexpr (NPL >= 2) *reinterpret_cast<half8*>(dst + plane_off(NPL, ldp, sp, 1, rl, c)) = mm;
+            if constexpr (NPL == 3) *reinterpret_cast<half8*>(dst + plane_off(NPL, ldp, sp, 2, rl, c)) = ll;
         }
     }
     // the tiles read these planes with direct-to-LDS loads: stores complete, then the
@@ -143,8 +143,11 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
     const int rows = min(kPanelRows, I.m - i0);
     _Float16* scr = reinterpret_cast<_Float16*>(static_cast<char*>(S.base) +
                                                 (long long)blockIdx.x * S.bytes_per_wg);
-    // the scratch holds rows i0..i0+63: row i of the operand sits at scr + (i - i0)*ldp
-    const _Float16* scrA = scr - (long long)i0 * S.ldp;
+    // the scratch holds rows i0..i0+63: row i of the operand sits (i - i0) plane rows
+    // after scr, in the layout of the step's plane count
+    auto scr_at = [&](int npl) -> const _Float16* {
+        return scr - (long long)i0 * plane_row_stride(npl, S.ldp);
+    };
     auto* t = reinterpret_cast<float (*)[65]>(raw);
     for (int q = 0; q < I.nsteps; ++q) {
         const BatchStep st = steps[I.step0 + q];
@@ -170,10 +173,10 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
                                         S.ldp, S.sp, t);
             P.n = s;
             P.k = s;
-            P.A = scrA;
+            P.A = scr_at(ANPL);
             P.lda = S.ldp;
             P.spA = S.sp;
-            P.B = static_cast<const _Float16*>(I.Zp) + (long long)st.eoff * kZpLd;
+            P.B = static_cast<const _Float16*>(I.Zp) + (long long)st.eoff * plane_row_stride(ANPL, kZpLd);
             P.ldb = kZpLd;
             P.spB = I.spZ;
             P.srcA = P.srcB = kPP;
@@ -199,8 +202,7 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
             // shifted by kshift1 when k0 is not a multiple of 8); past kw they meet the
             // zero padding of X1's planes
             const bool sh = (k0 & 7) != 0;
-            const long long kb = (long long)c0 * I.ldpL + k0 + (sh ? I.kshift1 : 0);
-            P.A = scrA;
+            const long long kc = (long long)k0 + (sh ? I.kshift1 : 0);
             P.lda = S.ldp;
             P.spA = S.sp;
             P.ldb = I.ldpL;
@@ -208,13 +210,21 @@ __device__ __forceinline__ void trsm_panel(const BatchInst& I, const BatchStep* 
             if (st.type == 1) {
                 panel_planes<UNPL, 64 * NW>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap,
                                             scr, S.ldp, S.sp, t);
-                P.B = static_cast<const _Float16*>(sh ? I.LpU1 : I.LpU) + kb;
+                const PlaneRef lr = plane_ref(UNPL, static_cast<const _Float16*>(sh ? I.LpU1 : I.LpU),
+                                              I.ldpL, c0, kc);
+                P.A = scr_at(UNPL);
+                P.B = lr.p;
+                P.kofsB = lr.kofs;
                 P.spB = I.spLU;
                 for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<UNPL, kPP, NW>(P, i0, j0, raw);
             } else {
                 panel_planes<ANPL, 64 * NW>(I.C + (long long)k0 * I.ldc, I.ldc, I.m, i0, kw, I.eSnap,
                                             scr, S.ldp, S.sp, t);
-                P.B = static_cast<const _Float16*>(sh ? I.LpA1 : I.LpA) + kb;
+                const PlaneRef lr = plane_ref(ANPL, static_cast<const _Float16*>(sh ? I.LpA1 : I.LpA),
+                                              I.ldpL, c0, kc);
+                P.A = scr_at(ANPL);
+                P.B = lr.p;
+                P.kofsB = lr.kofs;
                 P.spB = I.spLA;
                 for (int j0 = 0; j0 < w; j0 += kPanelCols) panel_tile<ANPL, kPP, NW>(P, i0, j0, raw);
             }

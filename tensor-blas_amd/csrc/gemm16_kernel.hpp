@@ -33,6 +33,7 @@
 
 #include "epilogue.hpp"
 #include "gemm_kernel.hpp"   // vector types, xcd_remap, kExpZeroDev
+#include "tb_kernels.hpp"    // plane layouts
 #include "tb_common.hpp"
 
 namespace tb {
@@ -41,7 +42,9 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 
 constexpr int kBK = 32;          // k per stage
 constexpr int kRowBytes = 64;    // one LDS row: 32 halves
-constexpr int kGroupM = 8;       // tile rows per group in the tile order of full problems
+// tile rows per group in the tile order of full problems (4 and 16 measured slower at
+// 8192^3 and in the TRSM 16384)
+constexpr int kGroupM = 8;
 
 __device__ __forceinline__ int swz(int row, int unit) { return unit ^ ((row >> 2) & 3); }
 
@@ -67,16 +70,31 @@ __device__ __forceinline__ void lds_dma16(const void* g, unsigned lds_base)
                  : "memory");
 }
 
-// One operand panel of R rows x kBK with NPL planes.
+// One operand panel of R rows x kBK with NPL planes.  LDS layout of a stage:
+//   separate planes (NPL 1, 3): plane p at p*PLANE_BYTES, rows of 64 bytes, unit u of row
+//     r at swz(r, u) (the 16-lane patterns below cover the 64 banks once);
+//   interleaved 2 planes (as in global memory, tb_kernels.hpp): rows of 128 bytes holding
+//     the 4 units of plane 0 then the 4 of plane 1, logical unit lu = 4p + u of row r at
+//     lu ^ ((r >> 1) & 7): 16 rows of one logical unit fall on 16 distinct 16-byte bank
+//     groups, so the fragment reads are conflict-free again.
 template <int NPL, int SRC, int R, int NT>
 struct Panel16 {
+    static constexpr bool IL = planes_il(NPL);
     static constexpr int UNITS = R * 4;                 // 16-byte units per plane
     static constexpr int PLANE_BYTES = R * kRowBytes;
     static constexpr int BYTES = NPL * PLANE_BYTES;
-    // direct-to-LDS: one instruction moves 64 units (1 KB) per wave
+    // direct-to-LDS: one instruction moves 64 units (1 KB) per wave: 16 rows of one plane
+    // (separate planes) or 8 rows of both planes (interleaved)
     static constexpr int DMA_PER_PLANE = UNITS / 64;    // wave-instructions per plane
     static constexpr int NWAVES = NT / 64;
     static constexpr int DMA_PER_WAVE = (NPL * DMA_PER_PLANE + NWAVES - 1) / NWAVES;
+
+    // LDS byte offset of 16-byte unit u (k 8u .. 8u+7) of plane p, row r of a stage
+    __device__ static __forceinline__ int lds_off(int r, int p, int u)
+    {
+        if constexpr (IL) return r * (2 * kRowBytes) + (((p << 2) | u) ^ ((r >> 1) & 7)) * 16;
+        else return p * PLANE_BYTES + r * kRowBytes + swz(r, u) * 16;
+    }
     // register staging (kSrcF32): ITEMS units of 8 k per thread
     static constexpr int ITEMS = (UNITS + NT - 1) / NT;
     static_assert(UNITS % 64 == 0, "rows");
@@ -90,38 +108,55 @@ struct Panel16 {
     int sc[SRC == kSrcF32 ? ITEMS : 1];
 
     // ---------------------------------------------------------------- planes
-    // This wave's share of a stage's direct-to-LDS loads: instruction q moves 16 plane
-    // rows (1 KB).  gp[q] is this lane's source at k = 0 (rows past `rows` clamped onto
-    // row rows-1: their C entries are never stored; k past the problem's K must hold
-    // zeros in the planes, which the host guarantees), ld[q] its LDS byte offset.
+    // This wave's share of a stage's direct-to-LDS loads: instruction q moves 1 KB.
+    // gp[q] is this lane's source at k = 0 (rows past `rows` clamped onto row rows-1:
+    // their C entries are never stored; k past the problem's K must hold zeros in the
+    // planes, which the host guarantees), ldo[q] its LDS byte offset.  kofs: the
+    // interleaved operand's column offset within the block P points at (a multiple of 8,
+    // so no 16-byte unit straddles two blocks).
     const _Float16* gp[SRC == kSrcPlanes ? DMA_PER_WAVE : 1];
     unsigned ldo[SRC == kSrcPlanes ? DMA_PER_WAVE : 1];
 
     __device__ __forceinline__ void dma_setup(const _Float16* __restrict__ P, long long ldp,
-                                             long long sp, int r0, int rows, int wave, int lane)
+                                             long long sp, int kofs, int r0, int rows, int wave,
+                                             int lane)
     {
         if constexpr (SRC == kSrcPlanes) {
 #pragma unroll
             for (int q = 0; q < DMA_PER_WAVE; ++q) {
-                const int inst = wave + q * NWAVES;          // 0 .. NPL*DMA_PER_PLANE-1
-                const int p = inst / DMA_PER_PLANE;
-                const int blk = inst % DMA_PER_PLANE;
-                const int pos = blk * 64 + lane;            // physical unit in the plane
-                const int rl = pos >> 2;
-                const int u = swz(rl, pos & 3);             // logical unit stored there
-                int r = r0 + rl;
-                r = r < rows ? r : rows - 1;
-                gp[q] = P + p * sp + (long long)r * ldp + u * 8;
-                ldo[q] = (unsigned)(p * PLANE_BYTES + blk * 1024);
+                const int inst = wave + q * NWAVES;
+                if constexpr (IL) {
+                    // 8 rows x 128 bytes: lane -> row blk*8 + lane/8, physical unit lane%8
+                    const int rl = inst * 8 + (lane >> 3);
+                    const int lu = (lane & 7) ^ ((rl >> 1) & 7);      // logical unit there
+                    const int p = lu >> 2, c = kofs + (lu & 3) * 8;
+                    int r = r0 + rl;
+                    r = r < rows ? r : rows - 1;
+                    gp[q] = P + (long long)r * 2 * ldp + (c >> 5) * 64 + p * 32 + (c & 31);
+                    ldo[q] = (unsigned)(inst * 1024);
+                } else {
+                    const int p = inst / DMA_PER_PLANE;             // 0 .. NPL*DMA_PER_PLANE-1
+                    const int blk = inst % DMA_PER_PLANE;
+                    const int pos = blk * 64 + lane;            // physical unit in the plane
+                    const int rl = pos >> 2;
+                    const int u = swz(rl, pos & 3);             // logical unit stored there
+                    int r = r0 + rl;
+                    r = r < rows ? r : rows - 1;
+                    gp[q] = P + p * sp + (long long)r * ldp + u * 8;
+                    ldo[q] = (unsigned)(p * PLANE_BYTES + blk * 1024);
+                }
             }
         }
     }
 
+    // the loads of the stage at k0 (a multiple of kBK): interleaved rows advance two
+    // planes' worth per 32 columns
     __device__ __forceinline__ void dma(int k0, unsigned lds_base) const
     {
         if constexpr (SRC == kSrcPlanes) {
+            const int adv = IL ? 2 * k0 : k0;
 #pragma unroll
-            for (int q = 0; q < DMA_PER_WAVE; ++q) lds_dma16(gp[q] + k0, lds_base + ldo[q]);
+            for (int q = 0; q < DMA_PER_WAVE; ++q) lds_dma16(gp[q] + adv, lds_base + ldo[q]);
         }
     }
 
@@ -212,7 +247,7 @@ struct Panel16 {
 #pragma unroll
             for (int q = 0; q < ITEMS; ++q) {
                 if (!live[q]) continue;
-                char* dst = lds + row[q] * kRowBytes + swz(row[q], unit[q]) * 16;
+                char* dst = lds + lds_off(row[q], 0, unit[q]);
                 if constexpr (NPL == 1) {
                     half8_t hh;
 #pragma unroll
@@ -231,8 +266,8 @@ struct Panel16 {
                         if constexpr (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
                     }
                     *reinterpret_cast<half8_t*>(dst) = hh;
-                    *reinterpret_cast<half8_t*>(dst + PLANE_BYTES) = mm;
-                    if constexpr (NPL == 3) *reinterpret_cast<half8_t*>(dst + 2 * PLANE_BYTES) = ll;
+                    *reinterpret_cast<half8_t*>(lds + lds_off(row[q], 1, unit[q])) = mm;
+                    if constexpr (NPL == 3) *reinterpret_cast<half8_t*>(lds + lds_off(row[q], 2, unit[q])) = ll;
                 }
             }
         }
@@ -361,8 +396,8 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
         int pair;
         const int k0 = stage_k0(s, pair);
         if (pair != dma_pair) {
-            pa.dma_setup(pair ? A2h : Ah, P.lda, P.spA, i0, M, wave, lane);
-            pb.dma_setup(pair ? B2h : Bh, P.ldb, P.spB, j0, N, wave, lane);
+            pa.dma_setup(pair ? A2h : Ah, P.lda, P.spA, P.kofsA, i0, M, wave, lane);
+            pb.dma_setup(pair ? B2h : Bh, P.ldb, P.spB, P.kofsB, j0, N, wave, lane);
             dma_pair = pair;
         }
         const unsigned base = smem_lds + (unsigned)((s % NST) * STAGE);
@@ -383,8 +418,8 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     for (int q = 0; q < (kLo ? NACC : 1); ++q) acc2[q] = floatx16_t{};
 
     // fragment addressing: lane reads row (fragment base + lane%32), 16-byte unit
-    // kk/8 + lane/32, swizzled by bits 2..3 of the row (the base is a multiple of 32)
-    const int frow = lane & 31, fsw = (frow >> 2) & 3, fu = lane >> 5;
+    // kk/8 + lane/32 (Panel16::lds_off applies the layout's swizzle)
+    const int frow = lane & 31, fu = lane >> 5;
     // fragments of the 16-wide k half `kk` of stage s, then their MFMAs
     struct Frags {
         half8_t a[FM][NPL], b[FN][NPL];
@@ -392,20 +427,21 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     auto load_frags = [&](int s, int kk, Frags& f) {
         const char* sA = smem + (s % NST) * STAGE;
         const char* sB = sA + PA::BYTES;
-        const int uo = ((kk >> 3) + fu) ^ fsw;
+        // fragment rows are 32-aligned bases + frow: the swizzles only see frow's bits
+        const int u = (kk >> 3) + fu;
 #pragma unroll
         for (int a = 0; a < FM; ++a) {
-            const int off = (wm * WTM + a * 32 + frow) * kRowBytes + uo * 16;
+            const int r = wm * WTM + a * 32 + frow;
 #pragma unroll
             for (int p = 0; p < NPL; ++p)
-                f.a[a][p] = *reinterpret_cast<const half8_t*>(sA + p * PA::PLANE_BYTES + off);
+                f.a[a][p] = *reinterpret_cast<const half8_t*>(sA + PA::lds_off(r, p, u));
         }
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
-            const int off = (wn * WTN + b * 32 + frow) * kRowBytes + uo * 16;
+            const int r = wn * WTN + b * 32 + frow;
 #pragma unroll
             for (int p = 0; p < NPL; ++p)
-                f.b[b][p] = *reinterpret_cast<const half8_t*>(sB + p * PB::PLANE_BYTES + off);
+                f.b[b][p] = *reinterpret_cast<const half8_t*>(sB + PB::lds_off(r, p, u));
         }
     };
     // dependent MFMAs on one accumulator are NACC issues apart

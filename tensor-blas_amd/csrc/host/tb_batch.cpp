@@ -490,7 +490,7 @@ int build(tb_handle_s* h, int tier, const tb_batch_entry* E, int count, long nb,
             add_rowexp(rj[1], b->rjblocks[1], Z + o.Z + z, s, kOpT, s * (long)(r - q), s,
                        (int*)I.eZ + ez);
             add_planes(Z + o.Z + z, s, kOpT, s * (long)(r - q), s, I.eZ + ez, b->anpl,
-                       planes + o.Zp + ez * kZpLd, kZpLd, I.spZ);
+                       planes + o.Zp + ez * plane_row_stride(b->anpl, kZpLd), kZpLd, I.spZ);
             z += s * s * (long long)(r - q);
             ez += s * (long long)(r - q);
             q = r;

@@ -579,7 +579,9 @@ struct TrsmCtx {
         // aligned and whose width is whole 32-column stages (the kernel copies planes
         // by stages and the planes continue past the block)
         if (l_planes() && tier == upd_tier && (k0 + kshift) % 8 == 0 && kw % 32 == 0) {
-            p.B = Lp.p + c0 * Lp.ldp + kshift + k0;
+            const PlaneRef lr = plane_ref(planes_of(h, tier), Lp.p, Lp.ldp, c0, kshift + k0);
+            p.B = lr.p;
+            p.kofsB = lr.kofs;
             p.ldb = Lp.ldp;
             p.spB = Lp.sp;
             p.srcB = kSrcPlanes;
@@ -615,7 +617,7 @@ struct TrsmCtx {
                                ldb, 1.0, 1.0, kCFull);
         p.cdiv = ldiag + off * es;   // X = (B + B W^T) D^-1
         if (acc_tier == kSplitF16) {
-            p.B = Zp.p + eoff[q] * Zp.ldp;
+            p.B = plane_ref(planes_of(h, acc_tier), Zp.p, Zp.ldp, eoff[q], 0).p;
             p.ldb = Zp.ldp;
             p.spB = Zp.sp;
             p.srcB = kSrcPlanes;
@@ -783,7 +785,9 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
             const float* zq = reinterpret_cast<const float*>(c.Z + c.zoff[q] * c.es);
             TB_HIP(rowexp(false, zq, s, kOpT, (int)(s * cnt), (int)s, 1, 0, ez, 0, st));
             TB_HIP(split_planes(planes_of(c.h, kSplitF16), zq, s, kOpT, (int)(s * cnt), (int)s, ez,
-                                c.Zp.p + (long long)c.eoff[q] * c.Zp.ldp, c.Zp.ldp, c.Zp.sp, 0, st));
+                                c.Zp.p + (long long)c.eoff[q] *
+                                             plane_row_stride(planes_of(c.h, kSplitF16), c.Zp.ldp),
+                                c.Zp.ldp, c.Zp.sp, 0, st));
         }
         q = r;
     }

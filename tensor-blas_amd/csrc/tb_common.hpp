@@ -71,6 +71,8 @@ struct GemmProb {
     int modeA, modeB, modeC;
     int srcA, srcB;            // FP16 tiers: 0 = FP32 matrix, 1 = K-contiguous FP16 planes
                                // (then A/B point at plane 0 and lda/ldb are the plane ld)
+    int kofsA, kofsB;          // interleaved 2-plane operands: column offset (< 32) of k = 0
+                               // from the 32-column block A/B point at (plane_ref)
     int tilesM, tilesN;        // tile grid of one batch element (tilesM == tilesN for kCLower)
     int tilesPer;              // tiles of one batch element
     int tileBegin;             // index of this problem's first tile in the launch

@@ -36,9 +36,41 @@ hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const G
 // XCD count, which the kernels' xcd_remap tile order relies on).
 int capped_grid(const void* kernel, int threads, int total, int max_cus);
 
+// FP16 plane layouts, element (plane p, row r, column c), c = kshift + k counted from
+// the row start:
+//   separate planes (plain: 1; split with 3):  p*sp + r*ldp + c
+//   the two split planes, interleaved per 32 columns:  r*2*ldp + (c/32)*64 + p*32 + c%32
+// so that one 32-column stage of a 2-plane row is one contiguous 128-byte segment: the
+// direct-to-LDS loads stream 128-byte segments about half again as fast as 64-byte ones
+// (MI355X, measured).  ldp (a multiple of 64) is the columns of one plane's row either
+// way, so both layouts take npl * rows * ldp halves.
+__host__ __device__ constexpr bool planes_il(int npl) { return npl == 2; }
+__host__ __device__ inline long long plane_row_stride(int npl, long long ldp)
+{
+    return planes_il(npl) ? 2 * ldp : ldp;
+}
+__host__ __device__ inline long long plane_off(int npl, long long ldp, long long sp, int p, long long r,
+                                               long long c)
+{
+    return planes_il(npl) ? r * 2 * ldp + (c >> 5) * 64 + p * 32 + (c & 31) : p * sp + r * ldp + c;
+}
+// A GEMM operand that starts at row r0 and column c0 of planes at base: the pointer the
+// GemmProb takes, and the column offset kofs (< 32, a multiple of 8) it takes with it
+// (interleaved planes only: their pointer stays on a 32-column block boundary).
+struct PlaneRef {
+    const _Float16* p;
+    int kofs;
+};
+__host__ __device__ inline PlaneRef plane_ref(int npl, const _Float16* base, long long ldp, long long r0,
+                                              long long c0)
+{
+    if (planes_il(npl)) return {base + r0 * 2 * ldp + (c0 >> 5) * 64, (int)(c0 & 31)};
+    return {base + r0 * ldp + c0, 0};
+}
+
 // Pre-conversion of the rows of op(X) (FP32, rowexp modes kOpN/kOpT/kOpSymL/kOpTriL,
 // rows x K) into npl K-contiguous FP16 planes: element (r, k) of plane p at
-// out[p*sp + r*ldp + kshift + k].  npl = 1: RNE fp16(x).  npl = 3: with
+// out[plane_off(npl, ldp, sp, p, r, kshift + k)].  npl = 1: RNE fp16(x).  npl = 3: with
 // xs = x * 2^(15 - e[r]) (e: row exponents, kExpZero -> 0), xs = h + 2^-11 m + 2^-22 l.
 // ldp must be a multiple of 64 with ldp >= kshift + K; entries of a row outside
 // [kshift, kshift + K) are written as zero.  e_used (optional, split tier): receives

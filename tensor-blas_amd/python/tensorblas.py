@@ -15,7 +15,8 @@ import os
 import torch  # noqa: F401  (must be loaded first: one HIP runtime per process)
 
 _HERE = os.path.dirname(os.path.realpath(__file__))
-LIB_PATH = os.path.join(_HERE, "..", "build", "libTensorBLAS.so")
+# the in-tree build; TENSORBLAS_LIB names another build of the same sources
+LIB_PATH = os.environ.get("TENSORBLAS_LIB") or os.path.join(_HERE, "..", "build", "libTensorBLAS.so")
 
 TIERS = {"plain": 0, "split": 1, "ozaki": 2, "fp32": 3, "fp64": 4}
 F64_TIERS = {"ozaki", "fp64"}
