@@ -31,8 +31,16 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     Arena a;
     const size_t oEA = a.reserve(sizeof(int) * m);
     const size_t oEB = a.reserve(sizeof(int) * n);
-    // slices hold rows padded to a multiple of 64
-    const size_t mp = (size_t)(m + 63) / 64 * 64, np = (size_t)(n + 63) / 64 * 64;
+    // slices hold rows padded to a multiple of 64; the wide kernel (128 x 64 tiles, eight
+    // waves) takes every K within one exactness chunk (SYR2K with beta = 0: one launch per
+    // product, the first product's FP64 sum kept in C), and its A slices cover a multiple
+    // of 128 rows
+    const long kchunk = ozaki_kchunk(S);
+    const bool wide = kpad <= kchunk && (!pairs || beta == 0.0) &&
+                      ozaki_wide_ok(S, S + h->ozaki_extra);
+    const long arows = wide ? (m + ozaki_wide_rows() - 1) / ozaki_wide_rows() * ozaki_wide_rows() : m;
+    const int rbA = (int)((arows + 63) / 64);
+    const size_t mp = (size_t)rbA * 64, np = (size_t)(n + 63) / 64 * 64;
     const size_t oSA = a.reserve((size_t)S * mp * kpad);
     const size_t oSB = a.reserve((size_t)S * np * kpad);
     const size_t oSA2 = pairs ? a.reserve((size_t)S * mp * kpad) : 0;
@@ -57,7 +65,7 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
             TB_HIP(rowexp(true, B2, lda, rmB, (int)n, (int)k, 1, 0, eB, 0, st));
         }
     }
-    TB_HIP(ozaki_slice(A, lda, modeA, (int)m, (int)k, (int)kpad, S, 1, 0, eA, sA, st));
+    TB_HIP(ozaki_slice(A, lda, modeA, (int)m, (int)k, (int)kpad, S, 1, 0, eA, sA, st, rbA));
     TB_HIP(ozaki_slice(B, ldb, rmB, (int)n, (int)k, (int)kpad, S, 1, 0, eB, sB, st));
     OzakiProb p{};
     p.sa = sA;
@@ -66,7 +74,7 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
         p.sa2 = reinterpret_cast<int8_t*>(ws + oSA2);
         p.sb2 = reinterpret_cast<int8_t*>(ws + oSB2);
         TB_HIP(ozaki_slice(A2, ldb, modeA, (int)m, (int)k, (int)kpad, S, 1, 0, eA,
-                           const_cast<int8_t*>(p.sa2), st));
+                           const_cast<int8_t*>(p.sa2), st, rbA));
         TB_HIP(ozaki_slice(B2, lda, rmB, (int)n, (int)k, (int)kpad, S, 1, 0, eB,
                            const_cast<int8_t*>(p.sb2), st));
     }
@@ -81,12 +89,27 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     p.kpad = (int)kpad;
     p.nslices = S;
     p.ngroups = S + h->ozaki_extra;
-    p.kchunk = (int)ozaki_kchunk(S);
+    p.kchunk = (int)kchunk;
     p.lowerC = lowerC ? 1 : 0;
+    p.wide = wide ? 1 : 0;
     const double flops = (lowerC ? (double)n * (n + 1) * k : 2.0 * m * n * k) * (pairs ? 2 : 1);
     hipEvent_t e0;
     TB_TRY(prof_begin(h, &e0));
-    TB_HIP(ozaki_gemm(p, st, h->max_cus));
+    if (wide && pairs) {
+        // the same sums in the same order as one launch over both products: the first
+        // product's D_g into FP64 (kept in C), the second's added onto it, then scaled
+        OzakiProb p2 = p;
+        p.sa2 = p.sb2 = nullptr;
+        p.partial = 1;
+        TB_HIP(ozaki_gemm(p, st, h->max_cus));
+        p2.sa = p2.sa2;
+        p2.sb = p2.sb2;
+        p2.sa2 = p2.sb2 = nullptr;
+        p2.partial = 2;
+        TB_HIP(ozaki_gemm(p2, st, h->max_cus));
+    } else {
+        TB_HIP(ozaki_gemm(p, st, h->max_cus));
+    }
     return prof_end(h, kProfOzakiGemm, e0, flops);
 }
 

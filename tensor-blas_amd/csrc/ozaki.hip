@@ -37,7 +37,8 @@ constexpr int kOzBlockK = 32;        // k per block of the slice layout (= the G
 // source read and the int8 write are coalesced.
 __global__ void __launch_bounds__(256)
 ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int rows, int K, int kpad,
-                   int S, long long sx, const int* __restrict__ exps, int8_t* __restrict__ out)
+                   int S, long long sx, const int* __restrict__ exps, int8_t* __restrict__ out,
+                   int rblocks)
 {
     __shared__ double t[kSliceTile][kSliceTile + 1];   // t[row][k]
     const int q = blockIdx.z;
@@ -87,7 +88,6 @@ ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int row
         }
         // panel-blocked layout: 32 k of all S slices of one 64-row block are contiguous
         const int k = k0 + kl0;
-        const long long rblocks = (rows + kSliceTile - 1) / kSliceTile;
         int8_t* dst = out + (long long)q * S * rblocks * kSliceTile * kpad +
                       (((long long)(k / kOzBlockK) * rblocks + blockIdx.y) * S + s) * (kSliceTile * kOzBlockK) +
                       rl * kOzBlockK + k % kOzBlockK;
@@ -320,17 +320,226 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
     }
 }
 
+// ---------------------------------------------------------------- wide tiles
+// The same products on 128 x 64 tiles of C with eight waves (two per SIMD, each a 32 x 32
+// block: one wave's stage bubbles -- fragment reads, the barrier -- are the other's MFMA
+// time) for K within one exactness chunk (kpad <= kchunk), where no FP64 accumulation
+// across chunks is kept in registers: the G int32 sums D_g are converted once, in
+// increasing g, as the chunked kernel's single chunk is, so the results are the same
+// bits.  SYR2K's second product is its own chunk there: a first launch stores the first
+// product's unscaled FP64 sums in C (partial = 1) and a second adds its D_g onto them in
+// the same order (partial = 2), so the two launches give the chunked kernel's bits.  A stage holds
+// 32 k of all S slices of the two 64-row blocks of A (adjacent in the panel-blocked
+// layout: A's slices are laid out with an even number of row blocks) and of one 64-row
+// block of B: 6 S KB, in the LDS layout above.  Per wave, the S A fragments stay in
+// registers and the B fragments are read one slice at a time.
+constexpr int OWM = 128;             // rows of a wide tile (two 64-row slice blocks)
+
+template <int S>
+constexpr int ozw_stages()
+{
+    return 4 * (6 * S * 1024) <= 160 * 1024 ? 4 : (3 * (6 * S * 1024) <= 160 * 1024 ? 3 : 2);
+}
+
+// Wide tile g (128 x 64) of the lower part of C, walked in bands of kBand tile rows:
+// inside a band column by column (rows fastest), first the columns left of the band's
+// diagonal (every row of the band), then those it crosses, column tj needing rows
+// ti >= tj / 2 (128 ti + 127 >= 64 tj).
+__device__ __forceinline__ void lower_wide_banded(int g, int T, int tn, int& ti, int& tj)
+{
+    int r0 = 0;
+    for (;;) {
+        const int gb = min(kBand, T - r0);
+        const int left = min(2 * r0, tn);
+        int cnt = left * gb;
+        for (int c = left; c < min(2 * (r0 + gb), tn); ++c) cnt += r0 + gb - c / 2;
+        if (g < cnt || r0 + gb >= T) break;
+        g -= cnt;
+        r0 += gb;
+    }
+    const int gb = min(kBand, T - r0);
+    const int left = min(2 * r0, tn);
+    if (g < left * gb) {
+        tj = g / gb;
+        ti = r0 + g % gb;
+        return;
+    }
+    g -= left * gb;
+    for (int c = left; c < min(2 * (r0 + gb), tn); ++c) {
+        const int first = c / 2 > r0 ? c / 2 : r0;
+        const int rows = r0 + gb - first;
+        if (g < rows) {
+            tj = c;
+            ti = first + g;
+            return;
+        }
+        g -= rows;
+    }
+    ti = T - 1;                                                   // unreachable for g < total
+    tj = tn - 1;
+}
+
+template <int S, int G>
+__global__ void __launch_bounds__(512)
+ozaki_gemm_wide_kernel(OzakiProb P, int tilesM, int tilesN, int totalTiles)
+{
+    constexpr int NST = ozw_stages<S>();
+    constexpr int PLANE = OBM * OBK;                 // one slice of one 64-row block: 2 KB
+    constexpr int A_BYTES = 2 * S * PLANE;
+    constexpr int STAGE = 3 * S * PLANE;
+    constexpr int NINST = STAGE / 1024;              // 1 KB direct-to-LDS instructions
+    constexpr int DMA_W = (NINST + 7) / 8;           // per wave (the last ones repeat)
+    constexpr int KEEP = (NST - 2) * DMA_W;
+    static_assert(NST >= 2 && NST * STAGE <= 160 * 1024 && KEEP <= 60, "stages");
+    __shared__ __attribute__((aligned(1024))) int8_t smem[NST * STAGE];
+
+#pragma unroll 1
+    for (int blk = blockIdx.x; blk < totalTiles; blk += gridDim.x) {
+        const int g = xcd_remap(blk, totalTiles);
+        int ti, tj;
+        if (P.lowerC) {
+            lower_wide_banded(g, tilesM, tilesN, ti, tj);
+        } else {
+            const int per = kBand * tilesN;
+            const int first = (g / per) * kBand;
+            const int gsz = min(tilesM - first, kBand);
+            const int in = g - (g / per) * per;
+            ti = first + in % gsz;
+            tj = in / gsz;
+        }
+        const int i0 = ti * OWM, j0 = tj * OBN;
+        const int tid = threadIdx.x, lane = tid & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int wm = wave & 3, wn = wave >> 2;
+
+        // instruction inst: A (inst < 4S): row block inst / 2S, slice, 1 KB half; B: slice,
+        // half.  Waves with fewer real instructions repeat their previous one (the same
+        // bytes to the same place), so every wave counts DMA_W per stage.
+        const int8_t* gsrc[DMA_W];
+        bool isB[DMA_W];
+        unsigned ldo[DMA_W];
+        const long long rbA = (long long)tilesM * 2;                 // A's row blocks (even)
+        const long long rbB = (long long)(P.n + OBN - 1) / OBN;
+        const long long kstepA = rbA * S * PLANE, kstepB = rbB * S * PLANE;
+        const unsigned smem_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)smem;
+        {
+            const int8_t* GA = P.sa;
+            const int8_t* GB = P.sb;
+#pragma unroll
+            for (int q = 0; q < DMA_W; ++q) {
+                int inst = wave + q * 8;
+                if (inst >= NINST) inst -= 8;
+                const bool b = inst >= 4 * S;
+                const int rest = b ? inst - 4 * S : inst;
+                const int rb = b ? 0 : rest / (2 * S);                  // A row block in the tile
+                const int sl = (rest % (2 * S)) / 2, half = rest % 2;
+                const int rl = half * 32 + (lane >> 1);                 // row within the 64-row block
+                const int u = (lane & 1) ^ ((rl >> 3) & 1);             // logical unit stored there
+                const long long blk64 = b ? tj : 2ll * ti + rb;         // 64-row block of the operand
+                gsrc[q] = (b ? GB : GA) + (blk64 * S + sl) * PLANE + rl * OBK + u * 16;
+                isB[q] = b;
+                ldo[q] = (unsigned)((b ? A_BYTES + sl * PLANE : (rb * S + sl) * PLANE) + half * 1024);
+            }
+        }
+        const int nblk = P.kpad / OBK;
+        auto issue = [&](int kb) {
+            const unsigned base = smem_lds + (unsigned)((kb % NST) * STAGE);
+#pragma unroll
+            for (int q = 0; q < DMA_W; ++q) oz_dma16(gsrc[q] + kb * (isB[q] ? kstepB : kstepA), base + ldo[q]);
+        };
+
+        intx16_t acc[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q] = intx16_t{};
+        // fragment of lane: row (block base + lane%32), logical unit lane/32
+        const int fr = lane & 31;
+        const int fo = fr * OBK + (((lane >> 5) ^ ((fr >> 3) & 1)) * 16);
+        const int offA = (wm >> 1) * S * PLANE + (wm & 1) * 32 * OBK + fo;
+        const int offB = A_BYTES + wn * 32 * OBK + fo;
+
+        if (nblk > 0) {
+            for (int b = 0; b < NST - 1; ++b)
+                if (b < nblk) issue(b);
+            const int issued = nblk < NST - 1 ? nblk : NST - 1;
+            if (issued == 3) TB_OZ_WAIT_BARRIER(2 * DMA_W);
+            else if (issued == 2) TB_OZ_WAIT_BARRIER(DMA_W);
+            else TB_OZ_WAIT_BARRIER(0);
+        }
+#pragma unroll 1
+        for (int b = 0; b < nblk; ++b) {
+            const int8_t* st = smem + (b % NST) * STAGE;
+            intx4_t af[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) af[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offA);
+            // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
+            const bool ahead = b + NST - 1 < nblk;
+            if (ahead) issue(b + NST - 1);
+            // D_{s+t} += A_s B_t (s, t < S; s + t < G): consecutive MFMAs update different D_g;
+            // int32 sums are exact, so their order does not matter
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                const intx4_t bt = *reinterpret_cast<const intx4_t*>(st + t * PLANE + offB);
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (s + t < G)
+                        acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bt, af[s], acc[s + t], 0, 0, 0);
+            }
+            if (ahead) TB_OZ_WAIT_BARRIER(KEEP);
+            else TB_OZ_WAIT_BARRIER(0);
+        }
+
+        // epilogue: the exactness chunk's D_g into FP64 in increasing g (on top of the
+        // first product's sum for partial = 2: the chunked kernel's order), then the
+        // scale, alpha and beta
+        const int i = i0 + wm * 32 + (lane & 31);
+        if (i < P.m) {
+            const int ea = P.ea[i];
+            const int sea = ea < -5000 ? 0 : ea;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = j0 + wn * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                if (j >= P.n || (P.lowerC && i < j)) continue;
+                double* c = P.C + (long long)i + (long long)j * P.ldc;
+                double f = P.partial == 2 ? *c : 0.0;
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg) f = f + __builtin_ldexp((double)acc[gg][r], -7 * (gg + 2));
+                if (P.partial == 1) {
+                    *c = f;
+                    continue;
+                }
+                const int eb = P.eb[j];
+                const int seb = eb < -5000 ? 0 : eb;
+                const double v = __builtin_ldexp(f, sea + seb);
+                *c = (P.beta == 0.0) ? P.alpha * v : __builtin_fma(P.alpha, v, P.beta * *c);
+            }
+        }
+        __syncthreads();      // the next tile refills the ring
+    }
+}
+
 }  // namespace
+
+int ozaki_wide_rows() { return OWM; }
+
+bool ozaki_wide_ok(int nslices, int ngroups)
+{
+    if (nslices < 3 || nslices > 10 || ngroups < nslices || ngroups > nslices + 1) return false;
+    return ngroups == nslices || (nslices >= 6 && nslices <= 9);
+}
 
 hipError_t ozaki_slice(const double* X, long long ld, int mode, int rows, int K, int kpad,
                        int nslices, int nb, long long sx, const int* exps, int8_t* slices,
-                       hipStream_t s)
+                       hipStream_t s, int rblocks)
 {
     if (rows <= 0 || kpad <= 0 || nb <= 0) return hipSuccess;
-    if (kpad % kSliceTile || nslices < kMinSlices || nslices > kMaxSlices) return hipErrorInvalidValue;
-    dim3 g(kpad / kSliceTile, (rows + kSliceTile - 1) / kSliceTile, nb);
+    const int need = (rows + kSliceTile - 1) / kSliceTile;
+    if (rblocks <= 0) rblocks = need;
+    if (kpad % kSliceTile || nslices < kMinSlices || nslices > kMaxSlices || rblocks < need)
+        return hipErrorInvalidValue;
+    // every row block of the layout is written (rows past `rows` as zero digits)
+    dim3 g(kpad / kSliceTile, rblocks, nb);
     hipLaunchKernelGGL(ozaki_slice_kernel, g, dim3(256), 0, s, X, ld, mode, rows, K, kpad, nslices,
-                       sx, exps, slices);
+                       sx, exps, slices, rblocks);
     return hipGetLastError();
 }
 
@@ -340,8 +549,38 @@ hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus)
     if (p.nslices < kMinSlices || p.nslices > kMaxSlices || p.kpad % OBK || p.kchunk < OBK ||
         p.kchunk % OBK || (p.ngroups != p.nslices && p.ngroups != p.nslices + 1))
         return hipErrorInvalidValue;
-    const int tm = (p.m + OBM - 1) / OBM, tn = (p.n + OBN - 1) / OBN;
     if (p.lowerC && (p.m != p.n)) return hipErrorInvalidValue;
+    if (p.wide) {
+        // one exactness chunk, one product (pairs are two launches, the partial sum kept
+        // in C); A's slices laid out with an even number of row blocks
+        if (p.kpad > p.kchunk || p.sa2 || p.partial < 0 || p.partial > 2 ||
+            (p.partial && p.beta != 0.0))
+            return hipErrorInvalidValue;
+        const int tm = (p.m + OWM - 1) / OWM, tn = (p.n + OBN - 1) / OBN;
+        int total = 0;
+        if (p.lowerC) {
+            for (int t = 0; t < tm; ++t) total += min(2 * t + 2, tn);
+        } else {
+            total = tm * tn;
+        }
+        const int key = p.nslices * 2 + (p.ngroups - p.nslices);
+        switch (key) {
+#define TB_OZW_CASE(S_, E_)                                                                       \
+    case S_ * 2 + E_:                                                                             \
+        hipLaunchKernelGGL((ozaki_gemm_wide_kernel<S_, S_ + E_>),                                 \
+                           dim3(capped_grid(reinterpret_cast<const void*>(ozaki_gemm_wide_kernel<S_, S_ + E_>), \
+                                            512, total, max_cus)),                                \
+                           dim3(512), 0, s, p, tm, tn, total);                                    \
+        break;
+            TB_OZW_CASE(3, 0) TB_OZW_CASE(4, 0) TB_OZW_CASE(5, 0) TB_OZW_CASE(6, 0) TB_OZW_CASE(7, 0)
+            TB_OZW_CASE(8, 0) TB_OZW_CASE(9, 0) TB_OZW_CASE(10, 0)
+            TB_OZW_CASE(6, 1) TB_OZW_CASE(7, 1) TB_OZW_CASE(8, 1) TB_OZW_CASE(9, 1)
+#undef TB_OZW_CASE
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    const int tm = (p.m + OBM - 1) / OBM, tn = (p.n + OBN - 1) / OBN;
     const int total = p.lowerC ? tm * (tm + 1) / 2 : tm * tn;
     const int key = p.nslices * 2 + (p.ngroups - p.nslices);
     switch (key) {

@@ -159,9 +159,16 @@ hipError_t add_matrix_f32(long long m, long long n, void* x, long long ldx, cons
 // so that one 32-k stage of a 64-row panel (all S slices) is 2048 S contiguous bytes.
 // exps[q*rows + r].  mode kOpN/kOpT/kOpSymL/kOpTriL as rowexp.  exps must be filled by
 // rowexp first.
+// rblocks: 64-row blocks of the layout (0: ceil(rows / 64)); blocks past the rows hold
+// zero digits.
 hipError_t ozaki_slice(const double* X, long long ld, int mode, int rows, int K, int kpad,
                        int nslices, int nb, long long sx, const int* exps, int8_t* slices,
-                       hipStream_t s);
+                       hipStream_t s, int rblocks = 0);
+// rows of a wide Ozaki tile: with OzakiProb::wide, op(A)'s slices must be laid out with
+// a multiple of this many rows (an even number of 64-row blocks)
+int ozaki_wide_rows();
+// the wide kernel exists for these slices / groups
+bool ozaki_wide_ok(int nslices, int ngroups);
 struct OzakiProb {
     const int8_t* sa;   // slices of op(A) rows   [kpad/32][ceil(m/64)][S][64][32]
     const int8_t* sb;   // slices of op(B) columns [kpad/32][ceil(n/64)][S][64][32]
@@ -175,6 +182,11 @@ struct OzakiProb {
     int m, n, kpad, nslices, kchunk;
     int ngroups;        // slice-pair groups g = 0 .. ngroups-1 kept: nslices or nslices + 1
     int lowerC;
+    int wide;           // 128 x 64 tiles on eight waves (needs kpad <= kchunk and op(A)'s
+                        // slices over a multiple of ozaki_wide_rows() rows); a SYR2K pair is
+                        // two launches of it, sa2 / sb2 unused:
+    int partial;        //   1: store the unscaled FP64 sum into C (the first product's chunk)
+                        //   2: add this chunk onto that sum, then scale (beta must be 0)
 };
 hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);
 

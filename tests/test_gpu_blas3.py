@@ -378,6 +378,56 @@ def test_ozaki_routines_bit_exact(h):
     assert np.array_equal(from_dev(dC3, m, nn, ldc2), R)
 
 
+@pytest.mark.parametrize("S,extra,n,k,beta", [
+    (8, 1, 300, 130, 0.0),       # one exactness chunk per product: two launches, sum kept in C
+    (8, 1, 300, 130, 0.5),       # beta != 0: both products in one launch of the chunked kernel
+    (9, 1, 257, 64, 0.0),
+    (10, 0, 129, 700, 0.0),
+])
+def test_ozaki_syr2k_paths_bit_exact(h, S, extra, n, k, beta):
+    g = rng(S * 7 + n + k)
+    A, B = spread(g, (n, k), 2.0), spread(g, (n, k), 2.0)
+    C0 = spread(g, (n, n), 0.5)
+    h.set_ozaki_slices(S)
+    h.set_ozaki_extra(extra)
+    dA, lda = to_dev(A, n + 3)
+    dB, ldb = to_dev(B, n + 1)
+    dC, ldc = to_dev(C0, n + 2)
+    h.syr2k("ozaki", n, k, 0.75, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(), ldc)
+    h.synchronize()
+    C = from_dev(dC, n, n, ldc)
+    h.set_ozaki_extra(0)
+    h.set_ozaki_slices(7)
+    # the two products as one concatenated K, each half zero-padded to a multiple of 64
+    kp = (k + 63) // 64 * 64
+    Ap = np.zeros((n, 2 * kp)); Ap[:, :k] = A; Ap[:, kp:kp + k] = B
+    Bp = np.zeros((n, 2 * kp)); Bp[:, :k] = B; Bp[:, kp:kp + k] = A
+    s1, e1 = _oz_slices(Ap, S)
+    s2, e2 = _oz_slices(Bp, S)
+    # each product is one exactness chunk (K within kchunk)
+    assert kp <= _ozaki_kchunk(S)
+    R = oracle.ozaki_gemm(s1, e1, s2, e2, 0.75, beta, C0, kchunk=kp, ngroups=S + extra)
+    assert np.array_equal(np.tril(C), np.tril(R)), np.max(np.abs(np.tril(C - R)))
+    assert np.array_equal(np.triu(C, 1), np.triu(C0, 1))
+
+
+def test_ozaki_syrk_past_one_chunk_bit_exact(h):
+    g = rng(515)
+    S, n, k = 10, 96, 13400
+    A = spread(g, (n, k), 0.5)
+    C0 = spread(g, (n, n), 0.5)
+    h.set_ozaki_slices(S)
+    dA, lda = to_dev(A)
+    dC, ldc = to_dev(C0)
+    h.syrk("ozaki", n, k, 1.0, dA.data_ptr(), lda, -1.0, dC.data_ptr(), ldc)
+    h.synchronize()
+    C = from_dev(dC, n, n, ldc)
+    h.set_ozaki_slices(7)
+    sa, ea = _oz_slices(A, S)
+    R = oracle.ozaki_gemm(sa, ea, sa, ea, 1.0, -1.0, C0, kchunk=_ozaki_kchunk(S))
+    assert np.array_equal(np.tril(C), np.tril(R)), np.max(np.abs(np.tril(C - R)))
+
+
 def test_ozaki_accuracy_vs_fp64(h):
     g = rng(0xC0F4)
     n, k = 256, 512
