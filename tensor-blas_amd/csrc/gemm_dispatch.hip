@@ -1,5 +1,6 @@
 // Instantiations of the grouped GEMM kernels and the host-side dispatcher.
 #include "gemm16_kernel.hpp"
+#include "gemm64_kernel.hpp"
 #include "gemm_kernel.hpp"
 #include "tb_kernels.hpp"
 
@@ -59,6 +60,21 @@ hipError_t run16_src(int shape, int srcA, int srcB, const GemmLaunch& L, hipStre
     return hipErrorInvalidValue;
 }
 
+// The direct-to-LDS FP64 kernel takes launches whose problems all read op(A) = A and
+// op(B) = B^T (panel rows contiguous) from 16-byte aligned operands with even leading
+// dimensions and batch strides (every 16-byte load then covers two whole elements).
+bool gemm64_ok(const GemmLaunch& L)
+{
+    auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    for (int q = 0; q < L.np; ++q) {
+        const GemmProb& p = L.p[q];
+        if (p.modeA != kOpN || p.modeB != kOpT || p.eOut || !al(p.A) || !al(p.B) ||
+            (p.A2 && (!al(p.A2) || !al(p.B2))) || (p.lda | p.ldb | p.sA | p.sB) & 1)
+            return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 int capped_grid(const void* kernel, int threads, int total, int max_cus)
@@ -100,6 +116,13 @@ hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const G
                                    : run<kFmaF32, 128, 128, 32, 2, 2>(L, s, max_cus);
     case kMfmaF64:
         if (srcA != kSrcF32 || srcB != kSrcF32) return hipErrorInvalidValue;
+        if (shape == kShapeSquare && gemm64_ok(L)) {
+            hipLaunchKernelGGL(gemm64_kernel,
+                               dim3(capped_grid(reinterpret_cast<const void*>(gemm64_kernel), g64::NT,
+                                                L.totalTiles, max_cus)),
+                               dim3(g64::NT), 0, s, L);
+            return hipGetLastError();
+        }
         return shape == kShapeLeaf ? run<kMfmaF64, 64, 256, 16, 1, 4>(L, s, max_cus)
                                    : run<kMfmaF64, 128, 128, 16, 2, 2>(L, s, max_cus);
     default:

@@ -102,6 +102,26 @@ def test_gemm(h, tier, ta, tb, m, n, k):
         assert_bound(C, R, S, k, u, extra_abs=rep, what=f"{tier} gemm")
 
 
+@pytest.mark.parametrize("m,n,k", [(2, 2, 1), (130, 258, 16), (256, 128, 17), (130, 66, 1000),
+                                   (384, 256, 48)])
+def test_gemm_fp64_nt_tight_ld(h, m, n, k):
+    """FP64 NT products with leading dimensions equal to the row counts (even, 16-byte
+    aligned columns): the operands end exactly where the matrices do."""
+    g = rng(m + 3 * n + 7 * k)
+    A, B = uniform(g, (m, k), dtype=np.float64), uniform(g, (n, k), dtype=np.float64)
+    C0 = uniform(g, (m, n), dtype=np.float64)
+    dA, lda = to_dev(A, m)
+    dB, ldb = to_dev(B, n)
+    dC, ldc = to_dev(C0, m)
+    h.gemm("fp64", 0, 1, m, n, k, -1.0, dA.data_ptr(), lda, dB.data_ptr(), ldb, 0.25,
+           dC.data_ptr(), ldc)
+    h.synchronize()
+    C = from_dev(dC, m, n, ldc)
+    R = -oracle.dgemm(A, B.T) + 0.25 * C0
+    S = absprod(A, B.T) + 0.25 * np.abs(C0)
+    assert_bound(C, R, S, k, U64, what="fp64 gemm nt")
+
+
 # ------------------------------------------------------------------- SYRK
 @pytest.mark.parametrize("tier", ["plain", "split", "fp32", "fp64"])
 @pytest.mark.parametrize("n,k", [(1, 1), (7, 33), (129, 64), (300, 257), (513, 40)])
