@@ -262,11 +262,15 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
             } else {
                 const float xs = __builtin_ldexpf(x, sc);
                 const _Float16 h = (_Float16)xs;
-                const float r1 = (xs - (float)h) * 2048.0f;
-                const _Float16 m = (_Float16)r1;
                 hh[j] = h;
-                mm[j] = m;
-                if (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
+                if (NPL == 2) {
+                    mm[j] = (_Float16)(xs - (float)h);          // xs - h exact in FP32
+                } else {
+                    const float r1 = (xs - (float)h) * 2048.0f;
+                    const _Float16 m = (_Float16)r1;
+                    mm[j] = m;
+                    ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
+                }
             }
         }
         const int c = c0 + cl0 + half * 8;

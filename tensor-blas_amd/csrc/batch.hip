@@ -115,11 +115,15 @@ __device__ __forceinline__ void panel_planes(const float* __restrict__ X, long l
                 } else {
                     const float xs = __builtin_ldexpf(x, sc);
                     const _Float16 h = (_Float16)xs;
-                    const float r1 = (xs - (float)h) * 2048.0f;
-                    const _Float16 mq = (_Float16)r1;
                     hh[j] = h;
-                    mm[j] = mq;
-                    if constexpr (NPL == 3) ll[j] = (_Float16)((r1 - (float)mq) * 2048.0f);
+                    if constexpr (NPL == 2) {
+                        mm[j] = (_Float16)(xs - (float)h);      // the 2-plane form: xs ~ h + m
+                    } else {
+                        const float r1 = (xs - (float)h) * 2048.0f;
+                        const _Float16 mq = (_Float16)r1;
+                        mm[j] = mq;
+                        ll[j] = (_Float16)((r1 - (float)mq) * 2048.0f);
+                    }
                 }
             }
             const int c = c0 + cl0 + half * 8;

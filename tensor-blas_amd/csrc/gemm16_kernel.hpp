@@ -19,8 +19,12 @@
 // and the six products of total weight >= 2^-22 go to three FP32 accumulators:
 //   acc0 += hh;  acc1 += hm + mh;  acc2 += hl + mm + lh;
 // C = (acc0 + 2^-11 (acc1 + 2^-11 acc2)) * 2^-(scA + scB).
-// NPL = 2 (split tier, two planes): xs = h + 2^-11 m + r with |r| <= 2^-22 |xs|max
-// (23 significant bits for the row's largest entries); acc0 += hh, acc1 += hm + mh.
+// NPL = 2 (split tier, two planes): xs = h + m + r, m = RNE fp16(xs - h) (so |m| <=
+// 2^-11 |xs|) and |r| <= 2^-22 |xs|max (23 significant bits for the row's largest
+// entries).  The three products go to one FP32 accumulator, acc0 += hh + hm + mh: every
+// product of two fp16 values is exact in FP32 and the accumulation rounds like an FP32
+// GEMM's, and one accumulator set leaves the registers for 128 x 64 wave tiles.
+// C = acc0 * 2^-(scA + scB).
 //
 // LDS: every plane of a panel is [row][32 halves] = 64-byte rows, unpadded; the four
 // 16-byte units of row r sit at unit position u ^ ((r >> 2) & 3).  Both the 16-lane
@@ -259,11 +263,15 @@ struct Panel16 {
                     for (int j = 0; j < 8; ++j) {
                         const float xs = __builtin_ldexpf(vf[q][j], sc[q]);
                         const _Float16 h = (_Float16)xs;
-                        const float r1 = (xs - (float)h) * 2048.0f;      // exact
-                        const _Float16 m = (_Float16)r1;
                         hh[j] = h;
-                        mm[j] = m;
-                        if constexpr (NPL == 3) ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
+                        if constexpr (NPL == 2) {
+                            mm[j] = (_Float16)(xs - (float)h);          // xs - h exact
+                        } else {
+                            const float r1 = (xs - (float)h) * 2048.0f;  // exact
+                            const _Float16 m = (_Float16)r1;
+                            mm[j] = m;
+                            ll[j] = (_Float16)((r1 - (float)m) * 2048.0f);
+                        }
                     }
                     *reinterpret_cast<half8_t*>(dst) = hh;
                     *reinterpret_cast<half8_t*>(lds + lds_off(row[q], 1, unit[q])) = mm;
@@ -408,12 +416,12 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     const int wm = wave % WM, wn = wave / WM;
     constexpr int NACC = FM * FN;
     floatx16_t acc0[NACC];
-    floatx16_t acc1[kSplit ? NACC : 1];
+    floatx16_t acc1[kLo ? NACC : 1];
     floatx16_t acc2[kLo ? NACC : 1];
 #pragma unroll
     for (int q = 0; q < NACC; ++q) acc0[q] = floatx16_t{};
 #pragma unroll
-    for (int q = 0; q < (kSplit ? NACC : 1); ++q) acc1[q] = floatx16_t{};
+    for (int q = 0; q < (kLo ? NACC : 1); ++q) acc1[q] = floatx16_t{};
 #pragma unroll
     for (int q = 0; q < (kLo ? NACC : 1); ++q) acc2[q] = floatx16_t{};
 
@@ -466,13 +474,14 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
             for (int q = 0; q < NACC; ++q)
                 acc2[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][2], f.a[q % FM][0], acc2[q], 0, 0, 0);
         } else if constexpr (kSplit) {
-            // two planes: hm + mh into acc1 (the 2^-22 products are not formed)
+            // two planes (m stored at its own scale, 2^-11 below h): hm + mh into acc0
+            // beside hh (the 2^-22 product mm is not formed)
 #pragma unroll
             for (int q = 0; q < NACC; ++q)
-                acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][1], acc1[q], 0, 0, 0);
+                acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][0], f.a[q % FM][1], acc0[q], 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < NACC; ++q)
-                acc1[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][1], f.a[q % FM][0], acc1[q], 0, 0, 0);
+                acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][1], f.a[q % FM][0], acc0[q], 0, 0, 0);
         }
     };
 
@@ -500,17 +509,26 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
             if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32)
                 if (more) load_regs(s + 1);
             const bool ahead = s + NST - 1 < nst;
-            // both halves' fragment reads go out before the first MFMA; the next
-            // stage's loads are issued between the halves, overlapping the MFMAs
+            // two stages: the other slot is free from the barrier on, so its loads go
+            // out first and have the whole stage to land
+            if constexpr (NST == 2)
+                if (ahead) issue_dma(s + 1);
+            // both halves' fragment reads go out before the first MFMA, except where the
+            // two halves' fragments and the accumulators would not fit the registers
+            // together (2 planes on 128 x 64 wave tiles): there the second half's reads
+            // follow the first half's MFMAs
+            constexpr bool kSeq = kSplit && WTM * WTN >= 128 * 64;
             Frags f0, f1;
             load_frags(s, 0, f0);
-            load_frags(s, 16, f1);
+            if constexpr (!kSeq) load_frags(s, 16, f1);
             // split tier: the wave in its MFMA block takes issue priority over the SIMD's
             // other wave (+2-3 % at 8192^3; the plain tier measured 3 % slower with it)
             if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
             mma(f0);
             if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
-            if (ahead) issue_dma(s + NST - 1);
+            if constexpr (NST > 2)
+                if (ahead) issue_dma(s + NST - 1);
+            if constexpr (kSeq) load_frags(s, 16, f1);
             if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
             mma(f1);
             if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
@@ -562,10 +580,11 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
                     const int jl = wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                     const int j = j0 + jl;
                     float v = acc0[q][r];
-                    if constexpr (kSplit) {
-                        float t = acc1[q][r];
-                        if constexpr (kLo) t = __builtin_fmaf(acc2[q][r], 1.0f / 2048.0f, t);
+                    if constexpr (kLo) {
+                        const float t = __builtin_fmaf(acc2[q][r], 1.0f / 2048.0f, acc1[q][r]);
                         v = __builtin_ldexpf(__builtin_fmaf(t, 1.0f / 2048.0f, v), -(scA + scB_s[jl]));
+                    } else if constexpr (kSplit) {
+                        v = __builtin_ldexpf(v, -(scA + scB_s[jl]));
                     }
                     if (in[u]) {
                         float* c = Cb + (long long)i + (long long)j * ldc;

@@ -43,7 +43,7 @@ hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s, int mc)
     if constexpr (NPL <= 2 && SA == kSrcPlanes && SB == kSrcPlanes) {
         if (shape == kShapeWide) {
             if constexpr (NPL == 1) return run16<NPL, SA, SB, 256, 256, 2, 4, 4>(L, s, 0);
-            return run16<NPL, SA, SB, 256, 128, 4, 2, 3>(L, s, 0);
+            else return run16<NPL, SA, SB, 256, 256, 2, 4, 2>(L, s, 0);
         }
     }
     if (shape == kShapeWide) return hipErrorInvalidValue;
@@ -91,7 +91,7 @@ int capped_grid(const void* kernel, int threads, int total, int max_cus)
 TileShape gemm_tile_shape(int tier, int shape)
 {
     if (shape == kShapeLeaf) return TileShape{64, 256};
-    if (shape == kShapeWide) return tier == kPlainF16 ? TileShape{256, 256} : TileShape{256, 128};
+    if (shape == kShapeWide) return TileShape{256, 256};
     return TileShape{128, 128};
 }
 

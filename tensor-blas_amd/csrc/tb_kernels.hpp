@@ -15,7 +15,7 @@ enum GemmShape : int {
     kShapeSquare = 0,  // 128 x 128 tiles: off-diagonal updates and whole routines
     kShapeLeaf = 1,    // 64 x 256 tiles: a workgroup owns all n <= 256 columns of its rows,
                        // so C may alias op(A) (in-place TRSM leaf X := B * inv(L)^T)
-    kShapeWide = 2,    // 256 x 256 tiles (plain) or 256 x 128 (2-plane split): full C,
+    kShapeWide = 2,    // 256 x 256 tiles (plain and 2-plane split): full C,
                        // both operands in FP16 planes; chosen by GemmQueue::flush for
                        // launches with enough tiles to fill the device
 };
@@ -70,8 +70,9 @@ __host__ __device__ inline PlaneRef plane_ref(int npl, const _Float16* base, lon
 
 // Pre-conversion of the rows of op(X) (FP32, rowexp modes kOpN/kOpT/kOpSymL/kOpTriL,
 // rows x K) into npl K-contiguous FP16 planes: element (r, k) of plane p at
-// out[plane_off(npl, ldp, sp, p, r, kshift + k)].  npl = 1: RNE fp16(x).  npl = 3: with
-// xs = x * 2^(15 - e[r]) (e: row exponents, kExpZero -> 0), xs = h + 2^-11 m + 2^-22 l.
+// out[plane_off(npl, ldp, sp, p, r, kshift + k)].  npl = 1: RNE fp16(x).  With
+// xs = x * 2^(15 - e[r]) (e: row exponents, kExpZero -> 0) and h = RNE fp16(xs):
+// npl = 2: m = RNE fp16(xs - h) (xs ~ h + m);  npl = 3: xs = h + 2^-11 m + 2^-22 l.
 // ldp must be a multiple of 64 with ldp >= kshift + K; entries of a row outside
 // [kshift, kshift + K) are written as zero.  e_used (optional, split tier): receives
 // e[r] as read, the exponents a GEMM must use with these planes when e keeps changing.
