@@ -35,6 +35,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "epilogue.hpp"
 #include "gemm_kernel.hpp"   // vector types, xcd_remap, kExpZeroDev
 #include "tb_kernels.hpp"    // plane layouts
@@ -106,7 +108,10 @@ struct Panel16 {
     // vmcnt value describes a stage for all of them
     static_assert((NPL * DMA_PER_PLANE) % NWAVES == 0, "even DMA split");
 
-    float vf[SRC == kSrcF32 ? ITEMS : 1][8];
+    // register stages of FP32 operands: loaded two iterations before they are stored
+    // (one where three planes' accumulators leave no room for a second)
+    static constexpr int RSTAGES = (SRC == kSrcF32 && NPL < 3) ? 2 : 1;
+    float vf[RSTAGES][SRC == kSrcF32 ? ITEMS : 1][8];
     int row[SRC == kSrcF32 ? ITEMS : 1], unit[SRC == kSrcF32 ? ITEMS : 1];
     bool live[SRC == kSrcF32 ? ITEMS : 1];
     int sc[SRC == kSrcF32 ? ITEMS : 1];
@@ -184,10 +189,12 @@ struct Panel16 {
         }
     }
 
+    template <int BUF>
     __device__ __forceinline__ void load_f32(const float* __restrict__ X, long long ld, int mode,
                                              bool isA, int r0, int k0, int rows, int K,
                                              bool vec_ok)
     {
+        float(&vf)[SRC == kSrcF32 ? ITEMS : 1][8] = this->vf[BUF % RSTAGES];
         if constexpr (SRC == kSrcF32) {
             int m = mode;
             bool zero = false, special = false;
@@ -245,8 +252,10 @@ struct Panel16 {
         }
     }
 
+    template <int BUF>
     __device__ __forceinline__ void store_f32(char* lds) const
     {
+        const float(&vf)[SRC == kSrcF32 ? ITEMS : 1][8] = this->vf[BUF % RSTAGES];
         if constexpr (SRC == kSrcF32) {
 #pragma unroll
             for (int q = 0; q < ITEMS; ++q) {
@@ -382,19 +391,21 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
         pair = s >= kblocks;
         return (s - pair * kblocks) * kBK;
     };
-    // FP32 operands of stage s into registers
-    auto load_regs = [&](int s) {
+    // FP32 operands of stage s into register stage `buf` (an integral constant)
+    auto load_regs = [&](auto buf, int s) {
+        constexpr int B = decltype(buf)::value;
         int pair;
         const int k0 = stage_k0(s, pair);
         if constexpr (SRCA == kSrcF32)
-            pa.load_f32(pair ? A2f : Af, P.lda, P.modeA, true, i0, k0, M, K, vecA);
+            pa.template load_f32<B>(pair ? A2f : Af, P.lda, P.modeA, true, i0, k0, M, K, vecA);
         if constexpr (SRCB == kSrcF32)
-            pb.load_f32(pair ? B2f : Bf, P.ldb, P.modeB, false, j0, k0, N, K, vecB);
+            pb.template load_f32<B>(pair ? B2f : Bf, P.ldb, P.modeB, false, j0, k0, N, K, vecB);
     };
-    auto store_regs = [&](int s) {
+    auto store_regs = [&](auto buf, int s) {
+        constexpr int B = decltype(buf)::value;
         char* base = smem + (s % NST) * STAGE;
-        pa.store_f32(base);
-        pb.store_f32(base + PA::BYTES);
+        pa.template store_f32<B>(base);
+        pb.template store_f32<B>(base + PA::BYTES);
     };
     // plane operands of stage s straight into LDS (sources re-derived when the second
     // product pair starts)
@@ -486,16 +497,64 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     };
 
     // ------------------------------------------------------------------ pipeline
-    // Stage s lives in LDS slot s % NST.  Iteration s: FP32 loads of s+1 (registers),
-    // direct-to-LDS loads of s+NST-1, MFMAs of s, FP32 stores of s+1, then wait until
-    // only the loads of stage s+NST-1 are outstanding (vmcnt(DMA_W)) and barrier: stage
-    // s+1 is complete in LDS for every wave, and slot s % NST is free to be reloaded.
+    // Stage s lives in LDS slot s % NST.  Iteration s: FP32 loads of stage s+D (D = 2,
+    // into register stage s % 2; D = 1 with three planes), direct-to-LDS loads of s+NST-1 (between the two MFMA halves;
+    // after the FP32 stores when there are FP32 operands; first thing with two stages),
+    // MFMAs of s, FP32 stores of stage s+1 from its register stage, then wait until
+    // only the loads of stages past s+1 are outstanding and barrier: stage s+1 is
+    // complete in LDS for every wave, and slot s % NST is free to be reloaded.
+    // With FP32 operands the register stores wait for every load issued before the
+    // newest register loads (the compiler cannot count the direct-to-LDS loads), so the
+    // direct loads go out after the stores and have the whole next iteration to land,
+    // and the register loads, issued D iterations ahead, D.
+    constexpr bool kRegs = SRCA == kSrcF32 || SRCB == kSrcF32;
+    constexpr int D = NPL < 3 ? 2 : 1;
+    using RS0 = std::integral_constant<int, 0>;
+    using RS1 = std::integral_constant<int, 1>;
+    auto iteration = [&](auto rs, int s) __attribute__((always_inline)) {
+        constexpr int R = decltype(rs)::value;                  // register stage of s+D
+        if constexpr (kRegs)
+            if (s + D < nst) load_regs(std::integral_constant<int, D == 2 ? R : 0>{}, s + D);
+        const bool ahead = s + NST - 1 < nst;
+        // two stages: the other slot is free from the barrier on, so its loads go out
+        // first and have the whole stage to land
+        if constexpr (NST == 2)
+            if (ahead) issue_dma(s + 1);
+        // both halves' fragment reads go out before the first MFMA, except where the
+        // two halves' fragments and the accumulators would not fit the registers
+        // together (2 planes on 128 x 64 wave tiles): there the second half's reads
+        // follow the first half's MFMAs
+        constexpr bool kSeq = kSplit && WTM * WTN >= 128 * 64;
+        Frags f0, f1;
+        load_frags(s, 0, f0);
+        if constexpr (!kSeq) load_frags(s, 16, f1);
+        // split tier: the wave in its MFMA block takes issue priority over the SIMD's
+        // other wave (+2-3 % at 8192^3; the plain tier measured 3 % slower with it)
+        if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
+        mma(f0);
+        if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
+        if constexpr (NST > 2 && !kRegs)
+            if (ahead) issue_dma(s + NST - 1);
+        if constexpr (kSeq) load_frags(s, 16, f1);
+        if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
+        mma(f1);
+        if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
+        if constexpr (kRegs)
+            if (s + 1 < nst) store_regs(std::integral_constant<int, D == 2 ? 1 - R : 0>{}, s + 1);
+        if constexpr (NST > 2 && kRegs)
+            if (ahead) issue_dma(s + NST - 1);
+        if (KEEP > 0 && ahead) TB_WAIT_VM_BARRIER(KEEP);
+        else TB_WAIT_VM_BARRIER(0);
+    };
     if (nst > 0) {
-        if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32) load_regs(0);
+        if constexpr (kRegs) {
+            load_regs(RS0{}, 0);
+            if (D == 2 && nst > 1) load_regs(RS1{}, 1);
+        }
 #pragma unroll
         for (int s = 0; s < NST - 1; ++s)
             if (s < nst) issue_dma(s);
-        if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32) store_regs(0);
+        if constexpr (kRegs) store_regs(RS0{}, 0);
         // stage 0 complete: only the other issued stages (1 .. issued-1) may still be
         // in flight, DMA_W loads each
         const int issued = nst < NST - 1 ? nst : NST - 1;
@@ -503,39 +562,17 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
         else if (issued == 3) TB_WAIT_VM_BARRIER(2 * DMA_W);
         else if (issued == 2) TB_WAIT_VM_BARRIER(DMA_W);
         else TB_WAIT_VM_BARRIER(0);
+        if constexpr (kRegs && D == 2) {
+            // register stages alternate: two iterations per trip keep their indices
+            // compile-time constants (stage s+D goes to register stage s % 2)
 #pragma unroll 1
-        for (int s = 0; s < nst; ++s) {
-            const bool more = s + 1 < nst;
-            if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32)
-                if (more) load_regs(s + 1);
-            const bool ahead = s + NST - 1 < nst;
-            // two stages: the other slot is free from the barrier on, so its loads go
-            // out first and have the whole stage to land
-            if constexpr (NST == 2)
-                if (ahead) issue_dma(s + 1);
-            // both halves' fragment reads go out before the first MFMA, except where the
-            // two halves' fragments and the accumulators would not fit the registers
-            // together (2 planes on 128 x 64 wave tiles): there the second half's reads
-            // follow the first half's MFMAs
-            constexpr bool kSeq = kSplit && WTM * WTN >= 128 * 64;
-            Frags f0, f1;
-            load_frags(s, 0, f0);
-            if constexpr (!kSeq) load_frags(s, 16, f1);
-            // split tier: the wave in its MFMA block takes issue priority over the SIMD's
-            // other wave (+2-3 % at 8192^3; the plain tier measured 3 % slower with it)
-            if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
-            mma(f0);
-            if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
-            if constexpr (NST > 2)
-                if (ahead) issue_dma(s + NST - 1);
-            if constexpr (kSeq) load_frags(s, 16, f1);
-            if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
-            mma(f1);
-            if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
-            if constexpr (SRCA == kSrcF32 || SRCB == kSrcF32)
-                if (more) store_regs(s + 1);
-            if (KEEP > 0 && ahead) TB_WAIT_VM_BARRIER(KEEP);
-            else TB_WAIT_VM_BARRIER(0);
+            for (int s = 0; s < nst; s += 2) {
+                iteration(RS0{}, s);
+                if (s + 1 < nst) iteration(RS1{}, s + 1);
+            }
+        } else {
+#pragma unroll 1
+            for (int s = 0; s < nst; ++s) iteration(RS0{}, s);
         }
     }
 
