@@ -1,8 +1,9 @@
 // Grouped FP16-MFMA GEMM for the plain and split tiers (gfx950, v_mfma_f32_32x32x16_f16).
 //
 // C = alpha * sum_pairs op(A) op(B) + beta * C over a flat list of output tiles of up to
-// kMaxProbs strided-batch problems (tb_common.hpp).  256 threads (4 waves, 64 x 64
-// accumulator tile each) own one 128 x 128 tile (64 x 256 for TRSM leaves) and walk K
+// kMaxProbs strided-batch problems (tb_common.hpp).  A workgroup of WM x WN waves owns
+// one BM x BN tile (128 x 128, 64 x 256 for TRSM leaves, 256 x 256 wide tiles: eight
+// waves of 128 x 64 in the plain tier, four of 128 x 128 in the split tier) and walks K
 // in BK = 32 steps through a ring of NST LDS stages.
 //
 // Operands arrive in one of two forms (template SRCA / SRCB):
@@ -166,6 +167,14 @@ struct Panel16 {
             const int adv = IL ? 2 * k0 : k0;
 #pragma unroll
             for (int q = 0; q < DMA_PER_WAVE; ++q) lds_dma16(gp[q] + adv, lds_base + ldo[q]);
+        }
+    }
+    // only load q of the stage at k0 (so that the loads can be spread between MFMAs)
+    __device__ __forceinline__ void dma_q(int q, int k0, unsigned lds_base) const
+    {
+        if constexpr (SRC == kSrcPlanes) {
+            const int adv = IL ? 2 * k0 : k0;
+            lds_dma16(gp[q] + adv, lds_base + ldo[q]);
         }
     }
 
@@ -546,7 +555,98 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
         if (KEEP > 0 && ahead) TB_WAIT_VM_BARRIER(KEEP);
         else TB_WAIT_VM_BARRIER(0);
     };
-    if (nst > 0) {
+
+    // Four waves (one per SIMD) of 128 x 128 accumulators on a 256 x 256 tile of plane
+    // operands.  With one wave per SIMD nothing else covers a gap in its MFMA stream,
+    // so each k half's MFMAs are issued one at a time with the other work placed
+    // between them: the other half's fragment reads, and after the stage's one barrier
+    // (between the two halves) the direct-to-LDS loads of stage s+NST into stage s's
+    // slot.  Every slot gets a stage, the last one again past the end, so every barrier
+    // waits for the same count: stage s+1 complete, NST-2 later stages in flight.  The
+    // MFMAs of each accumulator run in mma()'s order, so the bits equal the other tiles'.
+    constexpr bool kW4 = !kRegs && NT == 256 && BM == 256 && BN == 256;
+    if constexpr (kW4) {
+        static_assert(!kLo, "one or two planes");
+        constexpr int NPROD = kSplit ? 3 : 1;             // hh (, hm, mh)
+        constexpr int NM = NPROD * NACC;                  // MFMAs of one k half
+        constexpr int NRA = FM * NPL, NRB = FN * NPL;     // fragment reads of one k half
+        constexpr int NRD = NRA + NRB;
+        static_assert(NRD + DMA_W <= NM && 4 + NRD <= NM, "interleave");
+        auto mfma1 = [&](int j, const Frags& f) __attribute__((always_inline)) {
+            const int pr = j / NACC, q = j % NACC;
+            const int ua = pr == 1 ? 1 : 0, ub = pr == 2 ? 1 : 0;      // hh, hm, mh
+            acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][ub], f.a[q % FM][ua], acc0[q], 0, 0, 0);
+        };
+        auto read1 = [&](int i, int s, int kk, Frags& f) __attribute__((always_inline)) {
+            const char* sA = smem + (s % NST) * STAGE;
+            const int u = (kk >> 3) + fu;
+            if (i < NRA) {
+                const int a = i / NPL, p = i % NPL;
+                f.a[a][p] = *reinterpret_cast<const half8_t*>(sA + PA::lds_off(wm * WTM + a * 32 + frow, p, u));
+            } else {
+                const int b = (i - NRA) / NPL, p = (i - NRA) % NPL;
+                f.b[b][p] = *reinterpret_cast<const half8_t*>(sA + PA::BYTES + PB::lds_off(wn * WTN + b * 32 + frow, p, u));
+            }
+        };
+        // stage s's loads into LDS slot `slot` (sources re-derived when the second
+        // product pair starts)
+        auto setup_for = [&](int s, int& k0) {
+            int pair;
+            k0 = stage_k0(s, pair);
+            if (pair != dma_pair) {
+                pa.dma_setup(pair ? A2h : Ah, P.lda, P.spA, P.kofsA, i0, M, wave, lane);
+                pb.dma_setup(pair ? B2h : Bh, P.ldb, P.spB, P.kofsB, j0, N, wave, lane);
+                dma_pair = pair;
+            }
+        };
+        if (nst > 0) {
+#pragma unroll
+            for (int q = 0; q < NST; ++q) {
+                int k0;
+                setup_for(q < nst ? q : nst - 1, k0);
+                const unsigned base = smem_lds + (unsigned)(q * STAGE);
+                pa.dma(k0, base);
+                pb.dma(k0, base + PA::BYTES);
+            }
+            TB_WAIT_VM_BARRIER((NST - 1) * DMA_W);
+            Frags fa, fb;
+            load_frags(0, 0, fa);
+#pragma unroll 1
+            for (int s = 0; s < nst; ++s) {
+                // half 0 (fa); half 1's fragments (fb) read between its MFMAs
+#pragma unroll
+                for (int j = 0; j < NM; ++j) {
+                    mfma1(j, fa);
+                    if (j >= 4 && j < 4 + NRD) read1(j - 4, s, 16, fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                // stage s+1 complete for every wave, stage s read by every wave
+                TB_WAIT_VM_BARRIER(KEEP);
+                __builtin_amdgcn_sched_barrier(0);
+                // half 1 (fb); the next stage's half-0 fragments and stage s+NST's
+                // loads between its MFMAs
+                int k0;
+                setup_for(s + NST < nst ? s + NST : nst - 1, k0);
+                const unsigned base = smem_lds + (unsigned)((s % NST) * STAGE);
+                const int s1 = s + 1 < nst ? s + 1 : s;
+#pragma unroll
+                for (int j = 0; j < NM; ++j) {
+                    mfma1(j, fb);
+                    if (j < NRD) {
+                        read1(j, s1, 0, fa);
+                    } else if (j < NRD + DMA_W) {
+                        const int q = j - NRD;
+                        if (q < PA::DMA_PER_WAVE) pa.dma_q(q, k0, base);
+                        else pb.dma_q(q - PA::DMA_PER_WAVE, k0, base + PA::BYTES);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            // the loads past the end land in slots nothing reads, but must land before
+            // the workgroup's LDS is handed on
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+    } else if (nst > 0) {
         if constexpr (kRegs) {
             load_regs(RS0{}, 0);
             if (D == 2 && nst > 1) load_regs(RS1{}, 1);

@@ -31,11 +31,12 @@ hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s, int mc)
 {
     // a stage is (BM + BN) x 64 B per plane, and the ring holds as many stages (<= 4)
     // as fit in the 160 KB of LDS.  Eight waves (two per SIMD) where the registers
-    // allow.  Square, leaf and 2-plane wide tiles: 64 x 64 accumulators per wave.  The
-    // plain wide tile, 256 x 256, has 128 x 64 per wave: 25 % fewer LDS bytes per MFMA
-    // (+14 % at 8192^3 over 256 x 128); at 2 planes that tile's two accumulator sets
-    // would not fit two waves per SIMD, and four 128 x 64 waves on 256 x 128 measured
-    // no faster than eight 64 x 64 ones.
+    // allow.  Square and leaf tiles: 64 x 32 accumulators per wave.  Wide tiles are
+    // 256 x 256: eight waves of 128 x 64 in the plain tier (25 % fewer LDS bytes per
+    // MFMA than 256 x 128, +14 % at 8192^3), four of 128 x 128 in the split tier, whose
+    // MFMAs are issued one at a time with the fragment reads and the next stage's loads
+    // between them (+4-5 % at 8192^3 and 16384 x 8192 x 8192 over eight waves of
+    // 128 x 64, the same bits; the plain tier measured 6-35 % slower that way).
     if (shape == kShapeLeaf) {
         if constexpr (NPL == 2) return run16<NPL, SA, SB, 64, 256, 1, 8, 3>(L, s, mc);
         return run16<NPL, SA, SB, 64, 256, 1, 4, NPL == 3 ? 2 : 3>(L, s, mc);
@@ -43,7 +44,7 @@ hipError_t run16_shape(int shape, const GemmLaunch& L, hipStream_t s, int mc)
     if constexpr (NPL <= 2 && SA == kSrcPlanes && SB == kSrcPlanes) {
         if (shape == kShapeWide) {
             if constexpr (NPL == 1) return run16<NPL, SA, SB, 256, 256, 2, 4, 4>(L, s, 0);
-            else return run16<NPL, SA, SB, 256, 256, 2, 4, 2>(L, s, 0);
+            else return run16<NPL, SA, SB, 256, 256, 2, 2, 2>(L, s, 0);
         }
     }
     if (shape == kShapeWide) return hipErrorInvalidValue;
