@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "epilogue.hpp"
 #include "gemm_kernel.hpp"   // vector types, xcd_remap, kExpZeroDev
@@ -54,6 +55,18 @@ constexpr int kRowBytes = 64;    // one LDS row: 32 halves
 constexpr int kGroupM = 8;
 
 __device__ __forceinline__ int swz(int row, int unit) { return unit ^ ((row >> 2) & 3); }
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N-1>) in order
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // "s_waitcnt vmcnt(n)" then a workgroup barrier.  Written as one asm statement with a
 // memory clobber: the compiler neither moves memory operations across it nor adds the
@@ -572,19 +585,23 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
         constexpr int NRA = FM * NPL, NRB = FN * NPL;     // fragment reads of one k half
         constexpr int NRD = NRA + NRB;
         static_assert(NRD + DMA_W <= NM && 4 + NRD <= NM, "interleave");
-        auto mfma1 = [&](int j, const Frags& f) __attribute__((always_inline)) {
-            const int pr = j / NACC, q = j % NACC;
-            const int ua = pr == 1 ? 1 : 0, ub = pr == 2 ? 1 : 0;      // hh, hm, mh
+        // one MFMA / one fragment read of the flat lists, the index a compile-time
+        // constant (static_for below), so that the accumulators stay in registers
+        auto mfma1 = [&](auto jc, const Frags& f) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            constexpr int pr = j / NACC, q = j % NACC;
+            constexpr int ua = pr == 1 ? 1 : 0, ub = pr == 2 ? 1 : 0;   // hh, hm, mh
             acc0[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.b[q / FM][ub], f.a[q % FM][ua], acc0[q], 0, 0, 0);
         };
-        auto read1 = [&](int i, int s, int kk, Frags& f) __attribute__((always_inline)) {
+        auto read1 = [&](auto ic, int s, int kk, Frags& f) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
             const char* sA = smem + (s % NST) * STAGE;
             const int u = (kk >> 3) + fu;
-            if (i < NRA) {
-                const int a = i / NPL, p = i % NPL;
+            if constexpr (i < NRA) {
+                constexpr int a = i / NPL, p = i % NPL;
                 f.a[a][p] = *reinterpret_cast<const half8_t*>(sA + PA::lds_off(wm * WTM + a * 32 + frow, p, u));
             } else {
-                const int b = (i - NRA) / NPL, p = (i - NRA) % NPL;
+                constexpr int b = (i - NRA) / NPL, p = (i - NRA) % NPL;
                 f.b[b][p] = *reinterpret_cast<const half8_t*>(sA + PA::BYTES + PB::lds_off(wn * WTN + b * 32 + frow, p, u));
             }
         };
@@ -614,12 +631,12 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
 #pragma unroll 1
             for (int s = 0; s < nst; ++s) {
                 // half 0 (fa); half 1's fragments (fb) read between its MFMAs
-#pragma unroll
-                for (int j = 0; j < NM; ++j) {
-                    mfma1(j, fa);
-                    if (j >= 4 && j < 4 + NRD) read1(j - 4, s, 16, fb);
+                static_for<NM>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    mfma1(jc, fa);
+                    if constexpr (j >= 4 && j < 4 + NRD) read1(std::integral_constant<int, j - 4>{}, s, 16, fb);
                     __builtin_amdgcn_sched_barrier(0);
-                }
+                });
                 // stage s+1 complete for every wave, stage s read by every wave
                 TB_WAIT_VM_BARRIER(KEEP);
                 __builtin_amdgcn_sched_barrier(0);
@@ -629,18 +646,18 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
                 setup_for(s + NST < nst ? s + NST : nst - 1, k0);
                 const unsigned base = smem_lds + (unsigned)((s % NST) * STAGE);
                 const int s1 = s + 1 < nst ? s + 1 : s;
-#pragma unroll
-                for (int j = 0; j < NM; ++j) {
-                    mfma1(j, fb);
-                    if (j < NRD) {
-                        read1(j, s1, 0, fa);
-                    } else if (j < NRD + DMA_W) {
-                        const int q = j - NRD;
-                        if (q < PA::DMA_PER_WAVE) pa.dma_q(q, k0, base);
+                static_for<NM>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    mfma1(jc, fb);
+                    if constexpr (j < NRD) {
+                        read1(jc, s1, 0, fa);
+                    } else if constexpr (j < NRD + DMA_W) {
+                        constexpr int q = j - NRD;
+                        if constexpr (q < PA::DMA_PER_WAVE) pa.dma_q(q, k0, base);
                         else pb.dma_q(q - PA::DMA_PER_WAVE, k0, base + PA::BYTES);
                     }
                     __builtin_amdgcn_sched_barrier(0);
-                }
+                });
             }
             // the loads past the end land in slots nothing reads, but must land before
             // the workgroup's LDS is handed on
@@ -681,39 +698,41 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     const bool lowerC = (P.modeC == kCLower);
     const bool track = (P.eOut != nullptr);
     const long long ldc = P.ldc;
-#pragma unroll
-    for (int a = 0; a < FM; ++a) {
+    // every index into the accumulators is a compile-time constant (static_for), so that
+    // they stay in registers
+    static_for<FM>([&](auto ac) __attribute__((always_inline)) {
+        constexpr int a = decltype(ac)::value;
         const int il = wm * WTM + a * 32 + (lane & 31);
         const int i = i0 + il;
         int emax = kExpZeroDev;
         const int scA = kSplit ? scA_s[il] : 0;
-#pragma unroll
-        for (int b = 0; b < FN; ++b) {
-            const int q = b * FM + a;
+        static_for<FN>([&](auto bc) __attribute__((always_inline)) {
+            constexpr int b = decltype(bc)::value;
+            constexpr int q = b * FM + a;
             // each group's reads of C (and of the column scales) are all issued before
             // the first is used: behind per-element branches they went out one memory
             // latency apart.  Groups of 8; one at a time with three accumulator sets live,
             // whose registers leave no room for a group (the 3-plane batch kernels spill
             // even at 4).
             constexpr int G = kLo ? 1 : 8;
-#pragma unroll
-            for (int hf = 0; hf < 16 / G; ++hf) {
+            static_for<16 / G>([&](auto hc) __attribute__((always_inline)) {
+                constexpr int hf = decltype(hc)::value;
                 bool in[G];
                 float cold[G];
                 float cd[G];
                 const float* cdiv = static_cast<const float*>(P.cdiv);
-#pragma unroll
-                for (int u = 0; u < G; ++u) {
-                    const int r = hf * G + u;
+                static_for<G>([&](auto uc) __attribute__((always_inline)) {
+                    constexpr int u = decltype(uc)::value;
+                    constexpr int r = hf * G + u;
                     const int j = j0 + wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                     in[u] = i < M && j < N && (!lowerC || i >= j);
                     cold[u] = masked_load(Cb + (in[u] ? (long long)i + (long long)j * ldc : 0ll),
                                           in[u] && beta != 0.0f);
                     cd[u] = cdiv ? masked_load(cdiv + (in[u] ? j : 0), in[u]) : 1.0f;
-                }
-#pragma unroll
-                for (int u = 0; u < G; ++u) {
-                    const int r = hf * G + u;
+                });
+                static_for<G>([&](auto uc) __attribute__((always_inline)) {
+                    constexpr int u = decltype(uc)::value;
+                    constexpr int r = hf * G + u;
                     const int jl = wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                     const int j = j0 + jl;
                     float v = acc0[q][r];
@@ -738,11 +757,11 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
                             emax = e > emax ? e : emax;
                         }
                     }
-                }
-            }
-        }
+                });
+            });
+        });
         if (track && emax != kExpZeroDev) atomicMax(&emax_s[il], emax);
-    }
+    });
     if (track) {
         __syncthreads();
         for (int r = tid; r < BM; r += NT)

@@ -495,6 +495,12 @@ int symm(tb_handle_s* h, int tier, long m, long n, double alpha, const void* A, 
 namespace {
 
 constexpr long kLeafMax = 256;   // largest diagonal block the leaf kernels take
+// Updates whose solved block X1 is at least this wide read it from FP16 planes
+// converted once for the update; narrower ones read X1 in FP32 and convert it on its
+// way to LDS in every tile column (at most eight of them at m x 1024): a separate
+// conversion launch costs more than that there (at n = m = 16384, 57 of the 63
+// updates, 0.77 ms of conversions per solve on MI355X)
+constexpr long kXPlanesMinK = 2048;
 
 struct Leaf {
     long off, s;
@@ -585,9 +591,16 @@ struct TrsmCtx {
             p.ldb = Lp.ldp;
             p.spB = Lp.sp;
             p.srcB = kSrcPlanes;
-            // the solved block X1 is final: convert it once here rather than in every
-            // tile column of the product
-            TB_TRY(x_planes(tier, k0, kw, &p));
+            if (kw >= kXPlanesMinK) {
+                // the solved block X1 is final: convert it once here rather than in
+                // every tile column of the product
+                TB_TRY(x_planes(tier, k0, kw, &p));
+            } else if (tier == kSplitF16) {
+                // X1 in FP32, split in the tiles with a snapshot of its rows' exponents
+                // (this product's epilogues raise eBX meanwhile)
+                TB_HIP(hipMemcpyAsync(eX, eBX, sizeof(int) * m, hipMemcpyDeviceToDevice, h->stream));
+                p.eA = eX;
+            }
         } else if (tier == kSplitF16) {
             // FP32 operands: every tile scales X1's rows by eA, which this product's
             // epilogues raise meanwhile; read a snapshot so all tiles agree
