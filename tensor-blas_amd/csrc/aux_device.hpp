@@ -202,7 +202,7 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
                                                    int* __restrict__ e_used,
                                                    _Float16* __restrict__ out, long long ldp,
                                                    long long sp, int kshift, int bx, int by,
-                                                   float (*t)[65])
+                                                   float (*t)[65], bool lower_only = false)
 {
     const int r0 = by * 64, c0 = bx * 64;   // c: plane column = kshift + k
     const int tid = threadIdx.x;
@@ -217,6 +217,7 @@ __device__ __forceinline__ void split_planes_block(const float* __restrict__ X, 
         if (c0 + 63 - kshift <= r0) em = kOpN;
         else if (c0 - kshift > r0 + 63) em = mode == kOpSymL ? kOpT : -1;
     }
+    if (em < 0 && lower_only) return;      // wholly above the diagonal: nobody reads it
     const bool kmaj = em == kOpT;
     float v[16];
 #pragma unroll

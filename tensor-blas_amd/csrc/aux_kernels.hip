@@ -153,11 +153,12 @@ template <int NPL>
 __global__ void __launch_bounds__(256)
 split_planes_kernel(const float* __restrict__ X, long long ld, int mode, int rows, int K,
                     const int* __restrict__ e, int* __restrict__ e_used,
-                    _Float16* __restrict__ out, long long ldp, long long sp, int kshift)
+                    _Float16* __restrict__ out, long long ldp, long long sp, int kshift,
+                    bool lower_only)
 {
     __shared__ float t[64][65];
     split_planes_block<NPL>(X, ld, mode, rows, K, e, e_used, out, ldp, sp, kshift, blockIdx.x,
-                            blockIdx.y, t);
+                            blockIdx.y, t, lower_only);
 }
 
 dim3 grid2d(long long m, long long n)
@@ -174,7 +175,7 @@ dim3 grid2d(long long m, long long n)
 
 hipError_t split_planes(int npl, const float* X, long long ld, int mode, int rows, int K,
                         const int* e, _Float16* out, long long ldp, long long sp, int kshift,
-                        hipStream_t s, int* e_used)
+                        hipStream_t s, int* e_used, bool lower_only)
 {
     if (rows <= 0) return hipSuccess;
     if (ldp % 64 || kshift < 0 || ldp < kshift + K || npl < 1 || npl > 3 ||
@@ -183,13 +184,13 @@ hipError_t split_planes(int npl, const float* X, long long ld, int mode, int row
     dim3 g((unsigned)(ldp / 64), (unsigned)((rows + 63) / 64));
     if (npl == 1)
         hipLaunchKernelGGL(split_planes_kernel<1>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
-                           e_used, out, ldp, sp, kshift);
+                           e_used, out, ldp, sp, kshift, lower_only && mode == kOpTriL);
     else if (npl == 2)
         hipLaunchKernelGGL(split_planes_kernel<2>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
-                           e_used, out, ldp, sp, kshift);
+                           e_used, out, ldp, sp, kshift, lower_only && mode == kOpTriL);
     else
         hipLaunchKernelGGL(split_planes_kernel<3>, g, dim3(256), 0, s, X, ld, mode, rows, K, e,
-                           e_used, out, ldp, sp, kshift);
+                           e_used, out, ldp, sp, kshift, lower_only && mode == kOpTriL);
     return hipGetLastError();
 }
 

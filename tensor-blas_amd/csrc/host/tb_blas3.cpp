@@ -261,7 +261,8 @@ size_t planes_bytes(int npl, long rows, long K, int kshift = 0)
 }
 
 int make_planes(tb_handle_s* h, int npl, const float* X, long ld, int mode, long rows, long K,
-                const int* e, char* mem, Planes* out, int kshift = 0, int* e_used = nullptr)
+                const int* e, char* mem, Planes* out, int kshift = 0, int* e_used = nullptr,
+                bool lower_only = false)
 {
     out->p = reinterpret_cast<_Float16*>(mem);
     out->ldp = round_up(std::max(1l, K + kshift), 64);
@@ -269,7 +270,7 @@ int make_planes(tb_handle_s* h, int npl, const float* X, long ld, int mode, long
     hipEvent_t e0;
     TB_TRY(prof_begin(h, &e0));
     TB_HIP(split_planes(npl, X, ld, mode, (int)rows, (int)K, e, out->p, out->ldp, out->sp,
-                        kshift, h->stream, e_used));
+                        kshift, h->stream, e_used, lower_only));
     return prof_end(h, kProfPlanes, e0, (double)rows * (4.0 * K + 2.0 * npl * out->ldp));
 }
 
@@ -770,8 +771,10 @@ int trsm_run(TrsmCtx& c, const TrsmLayout& t, char* ws)
     }
     if (c.l_planes()) {
         const int npl = planes_of(c.h, c.upd_tier);
+        // every update reads L below its diagonal blocks (rows c0 .. c0+w-1 at columns
+        // k0 .. k0+kw-1 with k0 + kw <= c0): blocks above the diagonal stay unwritten
         TB_TRY(make_planes(c.h, npl, (const float*)c.A, c.lda, kOpTriL, c.n, c.n, c.eL,
-                           ws + t.oLp, &c.Lp, c.kshift));
+                           ws + t.oLp, &c.Lp, c.kshift, nullptr, true));
     }
     if (c.acc_tier == kSplitF16) {
         c.Zp.p = reinterpret_cast<_Float16*>(ws + t.oZp);

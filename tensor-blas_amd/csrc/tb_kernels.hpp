@@ -76,9 +76,11 @@ __host__ __device__ inline PlaneRef plane_ref(int npl, const _Float16* base, lon
 // ldp must be a multiple of 64 with ldp >= kshift + K; entries of a row outside
 // [kshift, kshift + K) are written as zero.  e_used (optional, split tier): receives
 // e[r] as read, the exponents a GEMM must use with these planes when e keeps changing.
+// lower_only (mode kOpTriL): 64 x 64 blocks wholly above the diagonal are left as they
+// are instead of written as zeros, for consumers that read only below the diagonal.
 hipError_t split_planes(int npl, const float* X, long long ld, int mode, int rows, int K,
                         const int* e, _Float16* out, long long ldp, long long sp, int kshift,
-                        hipStream_t s, int* e_used = nullptr);
+                        hipStream_t s, int* e_used = nullptr, bool lower_only = false);
 
 // -------------------------------------------------------------- row exponents
 // Sentinel exponent of an all-zero row.
