@@ -1018,13 +1018,13 @@ def main():
     ap.add_argument("--workload", default="trsm",
                     choices=["trsm", "syrk", "ozaki", "gemm", "copy", "batched", "serve", "potrf",
                              "trtri"])
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--budget-gib", type=float, default=32.0)
     ap.add_argument("--load", type=float, default=0.72)
     ap.add_argument("--large-ms", type=float, default=5.0)
     ap.add_argument("--max-large", type=int, default=1)
     ap.add_argument("--order", default="shortest", choices=["fifo", "shortest"])
-    ap.add_argument("--reserve-cus", type=int, default=32)
+    ap.add_argument("--reserve-cus", type=int, default=0)
     ap.add_argument("--partition", type=int, default=0, choices=[0, 1, 2],
                     help="serve: --reserve-cus CUs left free of large requests (0: GEMM CU caps, "
                          "1: CU-masked streams) or of small requests (2: CU-masked streams)")
