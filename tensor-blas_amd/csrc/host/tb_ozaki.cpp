@@ -32,11 +32,12 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     const size_t oEA = a.reserve(sizeof(int) * m);
     const size_t oEB = a.reserve(sizeof(int) * n);
     // slices hold rows padded to a multiple of 64; the wide kernel (128 x 64 tiles, eight
-    // waves) takes every K within one exactness chunk (SYR2K with beta = 0: one launch per
-    // product, the first product's FP64 sum kept in C), and its A slices cover a multiple
-    // of 128 rows
+    // waves) takes one exactness chunk of one product per launch (more than one with
+    // beta = 0: the FP64 sum of the chunks so far kept in C), and its A slices cover a
+    // multiple of 128 rows
     const long kchunk = ozaki_kchunk(S);
-    const bool wide = kpad <= kchunk && (!pairs || beta == 0.0) &&
+    const long nchunks = (kpad + kchunk - 1) / kchunk;
+    const bool wide = ((nchunks == 1 && !pairs) || beta == 0.0) &&
                       ozaki_wide_ok(S, S + h->ozaki_extra);
     const long arows = wide ? (m + ozaki_wide_rows() - 1) / ozaki_wide_rows() * ozaki_wide_rows() : m;
     const int rbA = (int)((arows + 63) / 64);
@@ -95,18 +96,25 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     const double flops = (lowerC ? (double)n * (n + 1) * k : 2.0 * m * n * k) * (pairs ? 2 : 1);
     hipEvent_t e0;
     TB_TRY(prof_begin(h, &e0));
-    if (wide && pairs) {
-        // the same sums in the same order as one launch over both products: the first
-        // product's D_g into FP64 (kept in C), the second's added onto it, then scaled
-        OzakiProb p2 = p;
-        p.sa2 = p.sb2 = nullptr;
-        p.partial = 1;
-        TB_HIP(ozaki_gemm(p, st, h->max_cus));
-        p2.sa = p2.sa2;
-        p2.sb = p2.sb2;
-        p2.sa2 = p2.sb2 = nullptr;
-        p2.partial = 2;
-        TB_HIP(ozaki_gemm(p2, st, h->max_cus));
+    if (wide && (pairs || nchunks > 1)) {
+        // the same sums in the same order as one chunked launch over both products: each
+        // chunk of each product one launch, the FP64 sum so far kept in C, scaled by the
+        // last.  The slices are blocked by 32 columns: chunk c of a product starts
+        // c * kchunk / 32 blocks in.
+        const long kstepA = (long)rbA * S * 64 * 32, kstepB = (long)(np / 64) * S * 64 * 32;
+        const int nl = (pairs ? 2 : 1) * (int)nchunks;
+        int li = 0;
+        for (int pp = 0; pp < (pairs ? 2 : 1); ++pp) {
+            for (long c = 0; c < nchunks; ++c, ++li) {
+                OzakiProb q = p;
+                q.sa = (pp ? p.sa2 : p.sa) + c * (kchunk / 32) * kstepA;
+                q.sb = (pp ? p.sb2 : p.sb) + c * (kchunk / 32) * kstepB;
+                q.sa2 = q.sb2 = nullptr;
+                q.kpad = (int)std::min(kchunk, kpad - c * kchunk);
+                q.partial = li == 0 ? 1 : (li == nl - 1 ? 2 : 3);
+                TB_HIP(ozaki_gemm(q, st, h->max_cus));
+            }
+        }
     } else {
         TB_HIP(ozaki_gemm(p, st, h->max_cus));
     }

@@ -326,9 +326,11 @@ ozaki_gemm_kernel(OzakiProb P, int tilesM, int totalTiles)
 // time) for K within one exactness chunk (kpad <= kchunk), where no FP64 accumulation
 // across chunks is kept in registers: the G int32 sums D_g are converted once, in
 // increasing g, as the chunked kernel's single chunk is, so the results are the same
-// bits.  SYR2K's second product is its own chunk there: a first launch stores the first
-// product's unscaled FP64 sums in C (partial = 1) and a second adds its D_g onto them in
-// the same order (partial = 2), so the two launches give the chunked kernel's bits.  A stage holds
+// bits.  Longer K and SYR2K's second product run as one launch per exactness chunk of
+// each product, in the chunked kernel's order: the first stores its unscaled FP64 sums
+// in C (partial = 1), middle ones add their D_g onto them (partial = 3), and the last
+// adds its D_g and applies the scale, alpha and beta = 0 (partial = 2), so the launches
+// give the chunked kernel's bits.  A stage holds
 // 32 k of all S slices of the two 64-row blocks of A (adjacent in the panel-blocked
 // layout: A's slices are laid out with an even number of row blocks) and of one 64-row
 // block of B: 6 S KB, in the LDS layout above.  Per wave, the S A fragments stay in
@@ -500,10 +502,10 @@ ozaki_gemm_wide_kernel(OzakiProb P, int tilesM, int tilesN, int totalTiles)
                 const int j = j0 + wn * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 if (j >= P.n || (P.lowerC && i < j)) continue;
                 double* c = P.C + (long long)i + (long long)j * P.ldc;
-                double f = P.partial == 2 ? *c : 0.0;
+                double f = P.partial >= 2 ? *c : 0.0;
 #pragma unroll
                 for (int gg = 0; gg < G; ++gg) f = f + __builtin_ldexp((double)acc[gg][r], -7 * (gg + 2));
-                if (P.partial == 1) {
+                if (P.partial == 1 || P.partial == 3) {
                     *c = f;
                     continue;
                 }
@@ -553,7 +555,7 @@ hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus)
     if (p.wide) {
         // one exactness chunk, one product (pairs are two launches, the partial sum kept
         // in C); A's slices laid out with an even number of row blocks
-        if (p.kpad > p.kchunk || p.sa2 || p.partial < 0 || p.partial > 2 ||
+        if (p.kpad > p.kchunk || p.sa2 || p.partial < 0 || p.partial > 3 ||
             (p.partial && p.beta != 0.0))
             return hipErrorInvalidValue;
         const int tm = (p.m + OWM - 1) / OWM, tn = (p.n + OBN - 1) / OBN;

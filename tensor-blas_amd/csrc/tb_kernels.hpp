@@ -186,9 +186,11 @@ struct OzakiProb {
     int ngroups;        // slice-pair groups g = 0 .. ngroups-1 kept: nslices or nslices + 1
     int lowerC;
     int wide;           // 128 x 64 tiles on eight waves (needs kpad <= kchunk and op(A)'s
-                        // slices over a multiple of ozaki_wide_rows() rows); a SYR2K pair is
-                        // two launches of it, sa2 / sb2 unused:
-    int partial;        //   1: store the unscaled FP64 sum into C (the first product's chunk)
+                        // slices over a multiple of ozaki_wide_rows() rows); longer K and a
+                        // SYR2K pair are one launch per chunk of each product, sa2 / sb2
+                        // unused:
+    int partial;        //   1: store the unscaled FP64 sum into C (the first chunk)
+                        //   3: add this chunk onto that sum (a middle chunk)
                         //   2: add this chunk onto that sum, then scale (beta must be 0)
 };
 hipError_t ozaki_gemm(const OzakiProb& p, hipStream_t s, int max_cus = 0);

@@ -335,6 +335,51 @@ def test_ozaki_gemm_bit_exact(h, S, extra, m, n, k, phi):
     assert np.array_equal(C, R), np.max(np.abs(C - R))
 
 
+@pytest.mark.parametrize("S,extra", [(9, 0), (8, 1)])
+def test_ozaki_long_k_bit_exact(h, S, extra):
+    """K past one exactness chunk with beta = 0 runs as one launch per chunk (each chunk's
+    FP64 sum kept in C between them): the chunked kernel's bits.  GEMM with three chunks,
+    SYR2K with two chunks of each product."""
+    kc = _ozaki_kchunk(S)
+    g = rng(9000 + S)
+    h.set_ozaki_slices(S)
+    h.set_ozaki_extra(extra)
+    try:
+        m, n, k = 70, 33, 2 * kc + 100
+        A, B = spread(g, (m, k), 0.5), spread(g, (k, n), 0.5)
+        dA, lda = to_dev(A)
+        dB, ldb = to_dev(B)
+        dC, ldc = to_dev(np.full((m, n), np.nan))
+        h.gemm("ozaki", 0, 0, m, n, k, 1.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, 0.0,
+               dC.data_ptr(), ldc)
+        h.synchronize()
+        sa, ea = _oz_slices(A, S)
+        sb, eb = _oz_slices(B.T.copy(), S)
+        R = oracle.ozaki_gemm(sa, ea, sb, eb, 1.5, 0.0, None, kchunk=_ozaki_kchunk(S), ngroups=S + extra)
+        C = from_dev(dC, m, n, ldc)
+        assert np.array_equal(C, R), np.max(np.abs(C - R))
+        # SYR2K: each product chunked at kchunk; the oracle's concatenated K pads each half
+        # to a multiple of kchunk so that the chunk boundaries fall in the same places
+        nn, k2 = 40, kc + 64
+        A2, B2 = spread(g, (nn, k2), 2.0), spread(g, (nn, k2), 2.0)
+        dA2, lda2 = to_dev(A2)
+        dB2, ldb2 = to_dev(B2)
+        dC2, ldc2 = to_dev(np.full((nn, nn), np.nan))
+        h.syr2k("ozaki", nn, k2, 1.0, dA2.data_ptr(), lda2, dB2.data_ptr(), ldb2, 0.0, dC2.data_ptr(), ldc2)
+        h.synchronize()
+        kp = 2 * _ozaki_kchunk(S)
+        Ap = np.zeros((nn, 2 * kp)); Ap[:, :k2] = A2; Ap[:, kp:kp + k2] = B2
+        Bp = np.zeros((nn, 2 * kp)); Bp[:, :k2] = B2; Bp[:, kp:kp + k2] = A2
+        s1, e1 = _oz_slices(Ap, S)
+        s2, e2 = _oz_slices(Bp, S)
+        R2 = oracle.ozaki_gemm(s1, e1, s2, e2, 1.0, 0.0, None, kchunk=_ozaki_kchunk(S), ngroups=S + extra)
+        C2 = from_dev(dC2, nn, nn, ldc2)
+        assert np.array_equal(np.tril(C2), np.tril(R2)), np.max(np.abs(np.tril(C2) - np.tril(R2)))
+    finally:
+        h.set_ozaki_extra(0)
+        h.set_ozaki_slices(7)
+
+
 def test_ozaki_extra_group_arguments(h, tbm):
     h.set_ozaki_slices(4)
     h.set_ozaki_extra(1)                      # accepted; the call then refuses nslices 4
