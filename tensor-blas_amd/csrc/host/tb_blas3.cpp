@@ -126,9 +126,9 @@ int GemmQueue::add(const GemmProb& p0)
     return TB_SUCCESS;
 }
 
-// Square launches of full-C plane products with at least one wide tile per CU run on
-// wide tiles instead (256 x 256 plain, 256 x 128 split: fewer operand bytes per MFMA);
-// smaller launches keep 128 x 128, whose more numerous tiles fill more of the device.
+// Square launches of plane products (full or lower-triangular C) with at least one wide
+// tile per CU run on 256 x 256 tiles instead (fewer operand bytes per MFMA); smaller
+// launches keep 128 x 128, whose more numerous tiles fill more of the device.
 bool GemmQueue::retile_wide()
 {
     const int npl = planes_of(h_, tier_);
@@ -137,18 +137,19 @@ bool GemmQueue::retile_wide()
         return false;
     const TileShape w = gemm_tile_shape(tier_, kShapeWide);
     long long tiles = 0;
-    for (int q = 0; q < L_.np; ++q) {
-        const GemmProb& p = L_.p[q];
-        if (p.modeC == kCLower) return false;
-        tiles += (long long)((p.m + w.bm - 1) / w.bm) * ((p.n + w.bn - 1) / w.bn) * p.batch;
-    }
+    // lower-triangular C (SYRK, SYR2K) takes the tiles on and below the diagonal
+    auto per = [&](const GemmProb& p) -> long long {
+        const long long tm = (p.m + w.bm - 1) / w.bm, tn = (p.n + w.bn - 1) / w.bn;
+        return p.modeC == kCLower ? tm * (tm + 1) / 2 : tm * tn;
+    };
+    for (int q = 0; q < L_.np; ++q) tiles += per(L_.p[q]) * L_.p[q].batch;
     if (tiles < kWideMinTiles) return false;
     int begin = 0;
     for (int q = 0; q < L_.np; ++q) {
         GemmProb& p = L_.p[q];
         p.tilesM = (p.m + w.bm - 1) / w.bm;
         p.tilesN = (p.n + w.bn - 1) / w.bn;
-        p.tilesPer = p.tilesM * p.tilesN;
+        p.tilesPer = (int)per(p);
         p.tileBegin = begin;
         begin += p.tilesPer * p.batch;
     }

@@ -134,3 +134,33 @@ def test_cu_cap_does_not_change_results(h, tier):
         hc.close()
     for a, b, what in zip(outs[0], outs[1], ("syrk", "trsm", "gemm")):
         assert torch.equal(a, b), f"{tier} {what}: {int((a != b).sum())} entries differ"
+
+
+@pytest.mark.parametrize("tier", ["plain", "split"])
+def test_syrk_wide_lower_equals_square(h, tier):
+    """SYRK and SYR2K at n = 5900: 24 x 25 / 2 = 300 lower-triangular wide tiles (>= 256),
+    so the product runs on 256 x 256 tiles; a handle capped to some CUs keeps 128 x 128
+    tiles.  Every element sums k in the same order: equal bit for bit, and the lower
+    triangle only is written."""
+    import tensorblas
+    g = rng(5900 + (tier == "split"))
+    n, k = 5900, 96
+    A = uniform(g, (n, k))
+    B = uniform(g, (n, k))
+    outs = []
+    for cap in (0, 64):
+        hc = tensorblas.Handle()
+        hc.set_max_cus(cap)
+        dA, lda = to_dev(A)
+        dB, ldb = to_dev(B)
+        dC, ldc = to_dev(np.full((n, n), np.nan, np.float32))
+        hc.syrk(tier, n, k, 1.0, dA.data_ptr(), lda, 0.0, dC.data_ptr(), ldc)
+        dD, _ = to_dev(np.full((n, n), np.nan, np.float32))
+        hc.syr2k(tier, n, k, 0.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, 0.0, dD.data_ptr(), ldc)
+        hc.synchronize()
+        outs.append((from_dev(dC, n, n, ldc), from_dev(dD, n, n, ldc)))
+        hc.close()
+    for a, b, what in zip(outs[0], outs[1], ("syrk", "syr2k")):
+        lo = np.tril_indices(n)
+        assert np.array_equal(a[lo], b[lo]), f"{what}: {int(np.sum(a[lo] != b[lo]))} entries differ"
+        assert np.isnan(np.triu(a, 1)[np.triu_indices(n, 1)]).all(), f"{what}: upper triangle written"
