@@ -97,7 +97,7 @@ __device__ __forceinline__ void lds_dma16(const void* g, unsigned lds_base)
 //     the 4 units of plane 0 then the 4 of plane 1, logical unit lu = 4p + u of row r at
 //     lu ^ ((r >> 1) & 7): 16 rows of one logical unit fall on 16 distinct 16-byte bank
 //     groups, so the fragment reads are conflict-free again.
-template <int NPL, int SRC, int R, int NT>
+template <int NPL, int SRC, int R, int NT, int RST = 0>
 struct Panel16 {
     static constexpr bool IL = planes_il(NPL);
     static constexpr int UNITS = R * 4;                 // 16-byte units per plane
@@ -123,8 +123,9 @@ struct Panel16 {
     static_assert((NPL * DMA_PER_PLANE) % NWAVES == 0, "even DMA split");
 
     // register stages of FP32 operands: loaded two iterations before they are stored
-    // (one where three planes' accumulators leave no room for a second)
-    static constexpr int RSTAGES = (SRC == kSrcF32 && NPL < 3) ? 2 : 1;
+    // (one where three planes' accumulators leave no room for a second), or RST (the
+    // TRSM leaves: every stage of K <= 256 at once)
+    static constexpr int RSTAGES = RST > 0 ? RST : ((SRC == kSrcF32 && NPL < 3) ? 2 : 1);
     float vf[RSTAGES][SRC == kSrcF32 ? ITEMS : 1][8];
     int row[SRC == kSrcF32 ? ITEMS : 1], unit[SRC == kSrcF32 ? ITEMS : 1];
     bool live[SRC == kSrcF32 ? ITEMS : 1];
@@ -317,7 +318,12 @@ struct Panel16 {
 template <int NPL, int SRCA, int SRCB, int BM, int BN, int WM, int WN, int NST>
 struct Gemm16Cfg {
     static constexpr int NT = 64 * WM * WN;
-    using PA = Panel16<NPL, SRCA, BM, NT>;
+    // TRSM leaf tiles (64 x 256, op(A) the leaf's own block of B in FP32, K <= 256): all
+    // of op(A)'s stages are loaded into registers before the first MFMA, so a leaf waits
+    // for one memory latency instead of one per stage
+    static constexpr int kLeafStages = 8;
+    static constexpr bool kLeafAll = SRCA == kSrcF32 && SRCB == kSrcPlanes && BM == 64 && BN == 256 && NPL <= 2;
+    using PA = Panel16<NPL, SRCA, BM, NT, kLeafAll ? kLeafStages : 0>;
     using PB = Panel16<NPL, SRCB, BN, NT>;
     static constexpr int STAGE = PA::BYTES + PB::BYTES;
     static constexpr int SMEM = NST * STAGE;        // LDS ring bytes
@@ -663,6 +669,43 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
             // the workgroup's LDS is handed on
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
+    } else if (CFG::kLeafAll && nst > 0 && nst <= CFG::kLeafStages) {
+        if constexpr (CFG::kLeafAll) {
+            static_for<CFG::kLeafStages>([&](auto sc) __attribute__((always_inline)) {
+                if (decltype(sc)::value < nst) load_regs(sc, decltype(sc)::value);
+            });
+#pragma unroll
+            for (int s = 0; s < NST - 1; ++s)
+                if (s < nst) issue_dma(s);
+            store_regs(RS0{}, 0);
+            // stage 0 complete (and every register load, issued before the stages)
+            const int issued = nst < NST - 1 ? nst : NST - 1;
+            if (issued == NST - 1) TB_WAIT_VM_BARRIER(KEEP);
+            else if (issued == 3) TB_WAIT_VM_BARRIER(2 * DMA_W);
+            else if (issued == 2) TB_WAIT_VM_BARRIER(DMA_W);
+            else TB_WAIT_VM_BARRIER(0);
+            static_for<CFG::kLeafStages>([&](auto sc) __attribute__((always_inline)) {
+                constexpr int s = decltype(sc)::value;
+                if (s < nst) {
+                    const bool ahead = s + NST - 1 < nst;
+                    if constexpr (NST == 2)
+                        if (ahead) issue_dma(s + 1);
+                    Frags f0, f1;
+                    load_frags(s, 0, f0);
+                    load_frags(s, 16, f1);
+                    if constexpr (kSplit) __builtin_amdgcn_s_setprio(1);
+                    mma(f0);
+                    mma(f1);
+                    if constexpr (kSplit) __builtin_amdgcn_s_setprio(0);
+                    if constexpr (s + 1 < CFG::kLeafStages)
+                        if (s + 1 < nst) store_regs(std::integral_constant<int, s + 1>{}, s + 1);
+                    if constexpr (NST > 2)
+                        if (ahead) issue_dma(s + NST - 1);
+                    if (KEEP > 0 && ahead) TB_WAIT_VM_BARRIER(KEEP);
+                    else TB_WAIT_VM_BARRIER(0);
+                }
+            });
+        }
     } else if (nst > 0) {
         if constexpr (kRegs) {
             load_regs(RS0{}, 0);
@@ -698,6 +741,31 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
     const bool lowerC = (P.modeC == kCLower);
     const bool track = (P.eOut != nullptr);
     const long long ldc = P.ldc;
+    // TRSM leaf tiles read every C value and column divisor of the lane before the first
+    // is used (the registers of op(A)'s stages are free by now): one memory latency
+    // instead of one per group
+    constexpr bool kPre = CFG::kLeafAll;
+    float pcold[kPre ? FM : 1][kPre ? FN : 1][16];
+    float pcd[kPre ? FN : 1][16];
+    if constexpr (kPre) {
+        const float* cdiv = static_cast<const float*>(P.cdiv);
+        static_for<FM>([&](auto ac) __attribute__((always_inline)) {
+            constexpr int a = decltype(ac)::value;
+            const int i = i0 + wm * WTM + a * 32 + (lane & 31);
+            static_for<FN>([&](auto bc) __attribute__((always_inline)) {
+                constexpr int b = decltype(bc)::value;
+                static_for<16>([&](auto rc) __attribute__((always_inline)) {
+                    constexpr int r = decltype(rc)::value;
+                    const int j = j0 + wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    const bool in = i < M && j < N && (!lowerC || i >= j);
+                    pcold[a][b][r] = masked_load(Cb + (in ? (long long)i + (long long)j * ldc : 0ll),
+                                                 in && beta != 0.0f);
+                    if constexpr (a == 0)
+                        pcd[b][r] = cdiv ? masked_load(cdiv + (j < N ? j : 0), j < N) : 1.0f;
+                });
+            });
+        });
+    }
     // every index into the accumulators is a compile-time constant (static_for), so that
     // they stay in registers
     static_for<FM>([&](auto ac) __attribute__((always_inline)) {
@@ -726,9 +794,14 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
                     constexpr int r = hf * G + u;
                     const int j = j0 + wn * WTN + b * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                     in[u] = i < M && j < N && (!lowerC || i >= j);
-                    cold[u] = masked_load(Cb + (in[u] ? (long long)i + (long long)j * ldc : 0ll),
-                                          in[u] && beta != 0.0f);
-                    cd[u] = cdiv ? masked_load(cdiv + (in[u] ? j : 0), in[u]) : 1.0f;
+                    if constexpr (kPre) {
+                        cold[u] = pcold[a][b][r];
+                        cd[u] = pcd[b][r];
+                    } else {
+                        cold[u] = masked_load(Cb + (in[u] ? (long long)i + (long long)j * ldc : 0ll),
+                                              in[u] && beta != 0.0f);
+                        cd[u] = cdiv ? masked_load(cdiv + (in[u] ? j : 0), in[u]) : 1.0f;
+                    }
                 });
                 static_for<G>([&](auto uc) __attribute__((always_inline)) {
                     constexpr int u = decltype(uc)::value;
