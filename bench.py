@@ -384,7 +384,7 @@ def run_potrf(args, torch, tensorblas):
                      flops / (ms * 1e-3) / 1e12, "TFLOP/s", args.steps, args.warmup, ms,
                      f"f32 ({args.split_planes}-plane fp16 split MFMA)",
                      "synthetic: A symmetric U[-1,1], diagonal + n (seeded)",
-                     {"workload": "POTRF A = L L^T lower, recursive (TRSM + SYRK + FP64 diagonal blocks)",
+                     {"workload": "POTRF A = L L^T lower, recursive (TRSM + SYRK + diagonal blocks in FP32)",
                       "n": n, "nb": nb})
     line.update({
         "tiers_tflops": {t: round(flops / (v * 1e-3) / 1e12, 2) for t, v in out.items()},
@@ -447,7 +447,7 @@ def run_trtri(args, torch, tensorblas):
                      flops / (ms * 1e-3) / 1e12, "TFLOP/s", args.steps, args.warmup, ms,
                      f"f32 ({args.split_planes}-plane fp16 split MFMA)",
                      "synthetic: L strictly lower U[-1,1], diagonal n + U[0,1] (seeded)",
-                     {"workload": "TRTRI inv(L) lower in place, recursive (TRMM + FP64 diagonal blocks)",
+                     {"workload": "TRTRI inv(L) lower in place, recursive (TRMM + diagonal blocks in FP32)",
                       "n": n, "nb": nb})
     line.update({
         "tiers_tflops": {t: round(flops / (v * 1e-3) / 1e12, 2) for t, v in out.items()},
@@ -876,14 +876,15 @@ def run_serve(args, torch, tensorblas):
     off = tensorblas.serve(slots, budget, [dict(r, arrival_us=0.0) for r in reqs])
     makespan = max(x["done_us"] for x in off)
     cap = count / makespan                                      # requests per microsecond
-    # served: Poisson arrivals at args.load of the offline capacity, two admission lanes
+    # served: Poisson arrivals at args.load of the offline capacity, three admission lanes
     restore()
     g = np.random.default_rng(0xC0F7 + 1)
     arr = np.cumsum(g.exponential(1.0 / (args.load * cap), count)) - 0.0
     arr -= arr[0]
     res = tensorblas.serve(slots, budget, [dict(r, arrival_us=float(a)) for r, a in zip(reqs, arr)],
                            large_us=args.large_ms * 1e3, max_large=args.max_large, order=args.order,
-                           reserve_cus=args.reserve_cus, partition=args.partition)
+                           reserve_cus=args.reserve_cus, partition=args.partition,
+                           small_us=args.small_ms * 1e3, small_slots=args.small_slots)
     lat = [x["done_us"] - a for x, a in zip(res, arr)]
     ratio = [l_ / i_ for l_, i_ in zip(lat, iso)]
     mix_sum = [checksum(torch, C) for _, _, C, _ in keep]
@@ -905,6 +906,7 @@ def run_serve(args, torch, tensorblas):
                       "hbm_budget_gib": args.budget_gib, "load": args.load,
                       "large_ms": args.large_ms, "max_large": args.max_large, "order": args.order,
                       "reserve_cus": args.reserve_cus, "partition": args.partition,
+                      "small_ms": args.small_ms, "small_slots": args.small_slots,
                       "total_flops": total_flops})
     line.update({
         "latency_ms": {"p50": round(pct(lat, 50) / 1e3, 3), "p99": round(pct(lat, 99) / 1e3, 3)},
@@ -1023,11 +1025,17 @@ def main():
     ap.add_argument("--load", type=float, default=0.72)
     ap.add_argument("--large-ms", type=float, default=5.0)
     ap.add_argument("--max-large", type=int, default=1)
-    ap.add_argument("--order", default="shortest", choices=["fifo", "shortest"])
+    ap.add_argument("--small-ms", type=float, default=0.0,
+                    help="serve: estimated time alone below which a request is small")
+    ap.add_argument("--small-slots", type=int, default=0,
+                    help="serve: slots kept for small requests")
+    ap.add_argument("--order", default="shortest", choices=["fifo", "shortest", "shortest-small"])
     ap.add_argument("--reserve-cus", type=int, default=0)
     ap.add_argument("--partition", type=int, default=0, choices=[0, 1, 2],
-                    help="serve: --reserve-cus CUs left free of large requests (0: GEMM CU caps, "
-                         "1: CU-masked streams) or of small requests (2: CU-masked streams)")
+                    help="serve: --reserve-cus CUs left free of large requests, or with "
+                         "--small-slots of every request outside the small lane (0: GEMM CU "
+                         "caps, 1: CU-masked streams), or of the other requests (2: CU-masked "
+                         "streams)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="serve: HIP hardware queues of the process (default: one per slot, <= 32)")
     ap.add_argument("--count", type=int, default=0)
@@ -1059,7 +1067,9 @@ def main():
         # One hardware queue per slot stream, set before HIP starts.  With fewer queues
         # than slots HIP maps several slot streams onto one queue, and a small request's
         # kernels then wait in FIFO order behind a large request's on another slot.
-        extra = {0: 0, 1: args.max_large, 2: args.slots}[args.partition]
+        upper = args.slots - args.small_slots if args.small_ms > 0 and args.small_slots > 0 \
+            else args.max_large
+        extra = {0: 0, 1: upper, 2: args.slots}[args.partition]
         args.hw_queues = min(32, max(1, args.hw_queues or args.slots + extra))
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
 
@@ -1089,7 +1099,8 @@ def main():
             "throughput_vs_offline": sv["throughput_vs_offline"],
             "results_equal_alone": sv["results_equal_alone"],
             "policy": {k: sv["config"][k] for k in ("slots", "load", "large_ms", "max_large", "order",
-                                                   "reserve_cus", "partition")},
+                                                   "reserve_cus", "partition", "small_ms",
+                                                   "small_slots")},
         }
     print(json.dumps(line), flush=True)
 

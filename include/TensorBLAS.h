@@ -272,10 +272,13 @@ TB_API int tb_batch_destroy(tb_batch_t batch);
 /* ------------------------------------------------------------- request loop
  * tb_serve runs a stream of requests -- any routine, any tier, each with its own
  * arrival time -- on nslots handles (each with its own stream and workspace) within
- * an HBM budget.  Requests go to one of two lanes by their estimated time alone
- * (flops over a per-tier rate): those above cfg->large_us form the large lane, of
- * which at most cfg->max_large run at once; the others never wait behind a large
- * request.  Within a lane a request is admitted first-come first-served as soon as a
+ * an HBM budget.  Requests go to one of three lanes by their estimated time alone
+ * (a per-routine, per-tier model): those above cfg->large_us form the large lane, of
+ * which at most cfg->max_large run at once; those below cfg->small_us the small lane,
+ * for which cfg->small_slots slots are kept (the other requests never hold more than
+ * nslots - small_slots slots); the rest the middle lane.  The lanes are served small
+ * first, then middle, then large, and no request waits behind one of a lane served
+ * after its own.  Within a lane a request is admitted first-come first-served as soon as a
  * slot is free and its bytes fit next to those already admitted; admitted requests
  * run concurrently on the device.  Operands are device pointers the caller keeps
  * resident.  Times are device times (HIP events) in microseconds from the start of
@@ -306,18 +309,25 @@ typedef struct {
     long hbm_budget;       /* bytes admitted requests may hold together */
     double large_us;       /* estimated time alone above which a request is large (0: one lane) */
     int max_large;         /* large requests running at once (>= 1) */
-    int order;             /* within a lane: TB_SERVE_FIFO (oldest waiting request first) or
-                              TB_SERVE_SHORTEST (shortest estimated time alone first) */
-    int reserve_cus;       /* compute units large requests leave free, so that small
-                              requests always find free CUs; 0: none */
-    int partition;         /* how they are left free: 0, large requests' GEMMs run under
-                              tb_set_max_cus; 1, large requests run on CU-masked streams
-                              (every kernel; needs GPU_MAX_HW_QUEUES >= nslots + max_large);
-                              2, the other way round: small requests run on CU-masked
-                              streams and the reserved CUs stay free for large ones
-                              (GPU_MAX_HW_QUEUES >= 2 nslots) */
+    int order;             /* within a lane: TB_SERVE_FIFO (oldest waiting request first),
+                              TB_SERVE_SHORTEST (shortest estimated time alone first) or
+                              TB_SERVE_SHORTEST_SMALL (shortest first in the small and
+                              middle lanes, oldest first in the large lane) */
+    int reserve_cus;       /* compute units the upper lanes leave free, so that the
+                              lower ones always find free CUs; 0: none.  Upper: the large
+                              lane, or, with small_slots > 0, every lane but the small one */
+    int partition;         /* how they are left free: 0, upper-lane requests' GEMMs run
+                              under tb_set_max_cus (the CUs left, shared by as many of them
+                              as may run at once); 1, upper-lane requests run on CU-masked
+                              streams (every kernel; needs GPU_MAX_HW_QUEUES >= nslots + as
+                              many); 2, the other way round: the lower lanes run on
+                              CU-masked streams and the reserved CUs stay free for the
+                              upper ones (GPU_MAX_HW_QUEUES >= 2 nslots) */
+    double small_us;       /* estimated time alone below which a request is small (0: no
+                              small lane) */
+    int small_slots;       /* slots kept for small requests (0 .. nslots - 1) */
 } tb_serve_config;
-enum { TB_SERVE_FIFO = 0, TB_SERVE_SHORTEST = 1 };
+enum { TB_SERVE_FIFO = 0, TB_SERVE_SHORTEST = 1, TB_SERVE_SHORTEST_SMALL = 2 };
 TB_API int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const tb_request* reqs,
              int count, tb_request_result* results);
 /* Grow a handle's two device workspaces (routines; Ozaki products) to at least these

@@ -4,30 +4,39 @@
 // completions are detected by polling events, and each request's times come from HIP
 // events against one start event, so they are device times.
 //
-// Admission: two lanes.  A request whose estimated time alone (a per-routine, per-tier
-// model, below) exceeds cfg->large_us is large; at most cfg->max_large large requests
-// run at once, so the longest requests do not share the device with each other, and
-// small requests never queue behind a large one.  Within a lane the next request --
-// the oldest waiting one (TB_SERVE_FIFO) or the one with the shortest estimate
-// (TB_SERVE_SHORTEST) -- is admitted as soon as a slot is free and the bytes it holds
-// fit the budget; no other request of its lane overtakes it.  Work submitted on one
-// slot stream never waits for another slot.
+// Admission: three lanes by the estimated time alone (a per-routine, per-tier model,
+// below).  A request above cfg->large_us is large; at most cfg->max_large large
+// requests run at once, so the longest requests do not share the device with each
+// other.  A request below cfg->small_us is small: cfg->small_slots slots are kept for
+// small requests, so at most nslots - small_slots of the others run at once and a
+// small request finds a slot without waiting for a long one to end.  The lanes are
+// served small first, then the rest, then large; no request waits behind one of a
+// lane served after its own.  Within a lane the next request -- the oldest waiting one
+// (TB_SERVE_FIFO), the one with the shortest estimate (TB_SERVE_SHORTEST), or the
+// shortest in the small and middle lanes and the oldest in the large lane
+// (TB_SERVE_SHORTEST_SMALL: a long request cannot be passed over indefinitely by
+// shorter ones that keep arriving) -- is admitted as soon as a slot is free and the
+// bytes it holds fit the budget; no other request of its lane overtakes it.  Work submitted on one slot stream never waits for
+// another slot.
 //
 // A kernel whose workgroups cannot all be resident keeps the CUs it is given as they
 // free up, so kernels from other streams start only once it has dispatched its last
 // workgroup: a small request admitted next to a large one would wait for most of it.
-// With cfg->reserve_cus > 0 large requests leave that many CUs to the small lane:
-//   partition 0: their GEMMs are capped (tb_set_max_cus) at (CUs - reserve_cus) /
-//     max_large CUs; the workgroups then all fit and loop over the tiles;
+// With cfg->reserve_cus > 0 the requests of the upper lanes -- the large lane; with
+// slots kept for a small lane, every request outside it -- leave that many CUs to the
+// lower ones:
+//   partition 0: their GEMMs are capped (tb_set_max_cus) at (CUs - reserve_cus) / (how
+//     many of them may run at once) CUs; the workgroups then all fit and loop over the
+//     tiles;
 //   partition 1: each runs on a stream created with a CU mask without the reserved
 //     CUs (hipExtStreamCreateWithCUMask), so none of its kernels -- GEMMs, conversions,
-//     slicing, inversions -- ever occupies them.  One such stream per concurrent large
-//     request, swapped in as the slot's stream for that request; each needs a hardware
-//     queue of its own (GPU_MAX_HW_QUEUES >= slots + max_large).
-// partition 2 turns the reservation around: small requests run on masked streams (one
+//     slicing, inversions -- ever occupies them.  One such stream per request that may
+//     run at once, swapped in as the slot's stream for that request; each needs a
+//     hardware queue of its own (GPU_MAX_HW_QUEUES >= slots + that many).
+// partition 2 turns the reservation around: the lower lanes run on masked streams (one
 // per slot; GPU_MAX_HW_QUEUES >= 2 slots) without the reserved CUs, which stay free for
-// the large lane.  Latency targets are absolute, so the longest requests have the least
-// slack: this keeps small requests from slowing them down.
+// the upper ones.  Latency targets are absolute, so the longest requests have the
+// least slack: this keeps small requests from slowing them down.
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -125,7 +134,7 @@ struct Running {
     int req;
     int slot;
     size_t bytes;
-    bool large;
+    int lane;         // 0 small, 1 the others, 2 large
     int mstream;      // index of the masked stream the request runs on, or -1
 };
 
@@ -152,21 +161,29 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
              int count, tb_request_result* out)
 {
     if (!slots || nslots <= 0 || !cfg || cfg->hbm_budget <= 0 || cfg->large_us < 0 ||
-        cfg->max_large < 1 || (cfg->order != TB_SERVE_FIFO && cfg->order != TB_SERVE_SHORTEST) ||
-        cfg->reserve_cus < 0 || cfg->partition < 0 || cfg->partition > 2 || count < 0 ||
+        cfg->max_large < 1 || cfg->order < TB_SERVE_FIFO || cfg->order > TB_SERVE_SHORTEST_SMALL ||
+        cfg->reserve_cus < 0 || cfg->partition < 0 || cfg->partition > 2 || cfg->small_us < 0 ||
+        cfg->small_slots < 0 || cfg->small_slots >= nslots || count < 0 ||
         (count > 0 && (!reqs || !out)))
         return TB_ERR_INVALID;
-    int large_cus = 0;                  // CU cap of a large request's GEMMs (0: none)
+    // lanes 0 small, 1 the others, 2 large; requests outside the small lane that may
+    // run at once; the first upper lane (the lanes that leave reserved CUs free) and
+    // how many upper-lane requests may run at once
+    const int max_regular = nslots - (cfg->small_us > 0 ? cfg->small_slots : 0);
+    const int upper = cfg->small_us > 0 && cfg->small_slots > 0 ? 1 : 2;
+    const int max_upper = upper == 1 ? max_regular : cfg->max_large;
+    int upper_cus = 0;                  // CU cap of an upper-lane request's GEMMs (0: none)
     int cus = 0;
     if (cfg->reserve_cus > 0) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, slots[0]->device) !=
             hipSuccess)
             return TB_ERR_DEVICE;
         if (cfg->reserve_cus >= cus) return TB_ERR_INVALID;
-        if (cfg->partition == 0) large_cus = std::max(1, (cus - cfg->reserve_cus) / cfg->max_large);
+        if (cfg->partition == 0) upper_cus = std::max(1, (cus - cfg->reserve_cus) / max_upper);
     }
-    // lane whose requests run on masked streams (-1: none)
-    const int masked_lane = cfg->reserve_cus > 0 && cfg->partition > 0 ? (cfg->partition == 1 ? 1 : 0) : -1;
+    // which lanes run on masked streams
+    const bool mask_on = cfg->reserve_cus > 0 && cfg->partition > 0;
+    auto masked = [&](int L) { return mask_on && (cfg->partition == 1 ? L >= upper : L < upper); };
     const size_t budget = (size_t)cfg->hbm_budget;
     for (int s = 0; s < nslots; ++s)
         if (!slots[s]) return TB_ERR_INVALID;
@@ -180,12 +197,14 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
     }
     if (count == 0) return TB_SUCCESS;
 
-    // the two lanes, each in arrival order
-    std::vector<int> lane[2];
+    // the three lanes, each in arrival order
+    std::vector<int> lane[3];
     std::vector<size_t> need(count);
     for (int i = 0; i < count; ++i) {
         need[i] = reqs[i].bytes > 0 ? (size_t)reqs[i].bytes : default_bytes(reqs[i]);
-        lane[cfg->large_us > 0 && out[i].est_us > cfg->large_us ? 1 : 0].push_back(i);
+        const double e = out[i].est_us;
+        lane[cfg->large_us > 0 && e > cfg->large_us ? 2 : (cfg->small_us > 0 && e < cfg->small_us ? 0 : 1)]
+            .push_back(i);
     }
     for (auto& l : lane)
         std::stable_sort(l.begin(), l.end(),
@@ -195,16 +214,16 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
     std::vector<hipEvent_t> done(nslots, nullptr);
     hipEvent_t t0 = nullptr;
     int st = TB_SUCCESS;
-    // masked streams (CUs reserve_cus .. cus-1) of the masked lane: max_large for the
-    // large lane, one per slot for the small lane
+    // masked streams (CUs reserve_cus .. cus-1): one per upper-lane request that may
+    // run at once (partition 1), or one per slot (partition 2)
     std::vector<hipStream_t> mstream;
     std::vector<int> mfree;
     std::vector<hipStream_t> own(nslots);
     for (int s = 0; s < nslots; ++s) own[s] = slots[s]->stream;
-    if (masked_lane >= 0) {
+    if (mask_on) {
         std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
         for (int c = cfg->reserve_cus; c < cus; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
-        const int nm = masked_lane == 1 ? cfg->max_large : nslots;
+        const int nm = cfg->partition == 1 ? max_upper : nslots;
         for (int q = 0; q < nm && st == TB_SUCCESS; ++q) {
             hipStream_t m = nullptr;
             if (hipExtStreamCreateWithCUMask(&m, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
@@ -231,9 +250,9 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
     for (int s = nslots - 1; s >= 0; --s) idle.push_back(s);
     std::vector<Running> run;
     size_t used = 0;
-    int large_running = 0, finished = 0;
-    size_t next[2] = {0, 0};           // next request of each lane still to arrive
-    std::vector<int> waiting[2];       // arrived, not admitted, in arrival order
+    int large_running = 0, regular_running = 0, finished = 0;
+    size_t next[3] = {0, 0, 0};        // next request of each lane still to arrive
+    std::vector<int> waiting[3];       // arrived, not admitted, in arrival order
     while (st == TB_SUCCESS && finished < count) {
         bool progress = false;
         // completions
@@ -250,7 +269,8 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
             }
             out[run[q].req].done_us = 1e3 * (double)ms;
             used -= run[q].bytes;
-            large_running -= run[q].large ? 1 : 0;
+            large_running -= run[q].lane == 2 ? 1 : 0;
+            regular_running -= run[q].lane > 0 ? 1 : 0;
             if (run[q].mstream >= 0) {               // the slot's own stream again
                 slots[run[q].slot]->stream = own[run[q].slot];
                 mfree.push_back(run[q].mstream);
@@ -261,10 +281,10 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
             ++finished;
             progress = true;
         }
-        // admissions, oldest first within each lane; the small lane first
+        // admissions, lane by lane: small, the others, large
         const double now_us =
             std::chrono::duration<double, std::micro>(Clock::now() - host0).count();
-        for (int L = 0; L < 2 && st == TB_SUCCESS; ++L) {
+        for (int L = 0; L < 3 && st == TB_SUCCESS; ++L) {
             while (next[L] < lane[L].size()) {
                 const int r = lane[L][next[L]];
                 if (reqs[r].arrival_us > now_us) break;
@@ -280,20 +300,21 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
             }
             while (!waiting[L].empty()) {
                 size_t w = 0;
-                if (cfg->order == TB_SERVE_SHORTEST)
+                if (cfg->order == TB_SERVE_SHORTEST || (cfg->order == TB_SERVE_SHORTEST_SMALL && L < 2))
                     for (size_t q = 1; q < waiting[L].size(); ++q)
                         if (out[waiting[L][q]].est_us < out[waiting[L][w]].est_us) w = q;
                 const int r = waiting[L][w];
                 if (idle.empty() || used + need[r] > budget) break;
-                if (L == 1 && large_running >= cfg->max_large) break;
+                if (L > 0 && regular_running >= max_regular) break;
+                if (L == 2 && large_running >= cfg->max_large) break;
                 const int s = idle.back();
                 idle.pop_back();
                 out[r].admit_us = now_us;
                 out[r].slot = s;
                 const int keep_cus = slots[s]->max_cus;
-                if (L == 1 && large_cus > 0) slots[s]->max_cus = large_cus;
+                if (L >= upper && upper_cus > 0) slots[s]->max_cus = upper_cus;
                 int ms = -1;
-                if (L == masked_lane) {        // a stream per request the lane may run
+                if (masked(L)) {               // a stream per request the lane may run
                     ms = mfree.back();
                     mfree.pop_back();
                     slots[s]->stream = mstream[(size_t)ms];
@@ -302,8 +323,9 @@ int tb_serve(tb_handle_t* slots, int nslots, const tb_serve_config* cfg, const t
                 slots[s]->max_cus = keep_cus;
                 if (hipEventRecord(done[s], slots[s]->stream) != hipSuccess) st = TB_ERR_DEVICE;
                 used += need[r];
-                large_running += L;
-                run.push_back({r, s, need[r], L == 1, ms});
+                large_running += L == 2 ? 1 : 0;
+                regular_running += L > 0 ? 1 : 0;
+                run.push_back({r, s, need[r], L, ms});
                 waiting[L].erase(waiting[L].begin() + (long)w);
                 progress = true;
             }

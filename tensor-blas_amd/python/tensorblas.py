@@ -347,25 +347,29 @@ class RequestResult(ctypes.Structure):
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_long), ("large_us", ctypes.c_double),
                 ("max_large", ctypes.c_int), ("order", ctypes.c_int), ("reserve_cus", ctypes.c_int),
-                ("partition", ctypes.c_int)]
+                ("partition", ctypes.c_int), ("small_us", ctypes.c_double),
+                ("small_slots", ctypes.c_int)]
 
 
-SERVE_ORDERS = {"fifo": 0, "shortest": 1}
+SERVE_ORDERS = {"fifo": 0, "shortest": 1, "shortest-small": 2}
 
 
 REQ_OPS = {"syrk": 0, "syr2k": 1, "trmm": 2, "trsm": 3, "symm": 4}
 
 
 def serve(slots, budget_bytes, requests, large_us=0.0, max_large=1, order="fifo", reserve_cus=0,
-          partition=0):
+          partition=0, small_us=0.0, small_slots=0):
     """Run a request stream (tb_serve) on the given handles; requests are dicts with
     op, tier, m, n, k, nb, alpha, beta, A, lda, B, ldb, C, ldc, arrival_us, bytes.
-    large_us > 0 splits admission into a small and a large lane (at most max_large
-    large requests at once); order "fifo" admits the oldest waiting request of a lane
-    first, "shortest" the one with the shortest estimated time alone; reserve_cus > 0
-    keeps that many CUs free of large requests: of their GEMMs (partition 0, CU caps) or of
-    all their kernels (partition 1, CU-masked streams); partition 2 keeps them free of small
-    requests instead.  Returns a list of dicts
+    large_us > 0 puts the requests estimated above it in a large lane (at most max_large
+    large requests at once); small_us > 0 those estimated below it in a small lane, for
+    which small_slots slots are kept; order "fifo" admits the oldest waiting request of
+    a lane first, "shortest" the one with the shortest estimated time alone,
+    "shortest-small" the shortest in the small and middle lanes and the oldest in the
+    large lane;
+    reserve_cus > 0 keeps that many CUs free of large requests: of their GEMMs
+    (partition 0, CU caps) or of all their kernels (partition 1, CU-masked streams);
+    partition 2 keeps them free of the other requests instead.  Returns a list of dicts
     admit_us, done_us, slot, status, est_us."""
     n = len(requests)
     arr = (Request * max(1, n))()
@@ -377,7 +381,7 @@ def serve(slots, budget_bytes, requests, large_us=0.0, max_large=1, order="fifo"
     res = (RequestResult * max(1, n))()
     hs = (ctypes.c_void_p * len(slots))(*[s.ptr.value for s in slots])
     cfg = ServeConfig(int(budget_bytes), float(large_us), int(max_large), SERVE_ORDERS[order],
-                      int(reserve_cus), int(partition))
+                      int(reserve_cus), int(partition), float(small_us), int(small_slots))
     _check("tb_serve", lib().tb_serve(hs, len(slots), ctypes.byref(cfg), arr, n, res))
     return [dict(admit_us=res[i].admit_us, done_us=res[i].done_us, slot=res[i].slot,
                  status=res[i].status, est_us=res[i].est_us) for i in range(n)]

@@ -113,6 +113,10 @@ def test_serve_rejects_bad_arguments(tbm):
                                      C=z.data_ptr(), ldc=1, arrival_us=-1.0)])
     with pytest.raises(tbm.TensorBLASError):
         tbm.serve(hs, 1 << 20, [], reserve_cus=32, partition=3)
+    with pytest.raises(tbm.TensorBLASError):          # every slot kept for small requests
+        tbm.serve(hs, 1 << 20, [], small_us=100.0, small_slots=1)
+    with pytest.raises(tbm.TensorBLASError):
+        tbm.serve(hs, 1 << 20, [], small_us=-1.0)
 
 
 @pytest.mark.parametrize("reserve_cus,partition", [(0, 0), (32, 0), (32, 1), (32, 2)])
@@ -162,3 +166,39 @@ def test_serve_shortest_first(tbm):
     assert by_admit == list(range(len(res)))
     for i, (_, _, C, _) in enumerate(keep):
         assert torch.equal(C, first[i]), (reqs[i]["op"], reqs[i]["tier"])
+
+
+@pytest.mark.parametrize("order,reserve_cus,partition",
+                         [("fifo", 0, 0), ("shortest", 0, 0), ("shortest-small", 0, 0), ("shortest", 32, 0),
+                          ("shortest", 32, 1), ("shortest", 32, 2)])
+def test_serve_small_lane(tbm, order, reserve_cus, partition):
+    """With small_slots slots kept for requests estimated below small_us, the other
+    requests never hold more than nslots - small_slots slots at once, small ones are
+    admitted before the others that wait with them (with reserve_cus, the others leave
+    CUs free for them, or run beside them on masked streams: partition 0, 1, 2), and
+    every result still equals the isolated one."""
+    reqs, keep = make_requests(17, sizes=(96, 384, 160))
+    alone, est = [], []
+    h1 = slots(tbm, 1)
+    for r, (_, _, C, _) in zip(reqs, keep):
+        est.append(tbm.serve(h1, 1 << 34, [dict(r, arrival_us=0.0)])[0]["est_us"])
+        alone.append(C.clone())
+    restore(keep)
+    small = float(np.percentile(est, 40))
+    large = float(np.percentile(est, 85))
+    hs = slots(tbm, 3)
+    res = tbm.serve(hs, 1 << 34, [dict(r, arrival_us=0.0) for r in reqs], large_us=large,
+                    max_large=1, order=order, small_us=small, small_slots=1,
+                    reserve_cus=reserve_cus, partition=partition)
+    assert all(x["status"] == 0 for x in res)
+    regular = [x for x in res if x["est_us"] >= small]
+    smalls = [x for x in res if x["est_us"] < small]
+    assert 0 < len(smalls) < len(res) and len(regular) > 2
+    for x in regular:            # at most 3 - 1 = 2 of them at any admission
+        busy = [y for y in regular if y is not x and y["admit_us"] <= x["admit_us"]
+                and y["done_us"] > x["admit_us"] + 200.0]
+        assert len(busy) <= 1, (x, busy)
+    # everything is waiting from t = 0 and the small lane is served first
+    assert max(x["admit_us"] for x in smalls) <= min(x["admit_us"] for x in regular)
+    for i, x in enumerate(res):
+        assert torch.equal(keep[i][2], alone[i]), (reqs[i]["op"], reqs[i]["tier"], reqs[i]["n"])
