@@ -327,6 +327,10 @@ struct Gemm16Cfg {
     using PB = Panel16<NPL, SRCB, BN, NT>;
     static constexpr int STAGE = PA::BYTES + PB::BYTES;
     static constexpr int SMEM = NST * STAGE;        // LDS ring bytes
+    // split-tier square tiles with an FP32 op(A) and two stages: two workgroups per CU
+    // (66 KB of LDS each), their registers held to 128 per lane (four waves per SIMD)
+    static constexpr int kWavesPerEU =
+        (NPL == 2 && SRCA == kSrcF32 && SRCB == kSrcPlanes && BM == 128 && BN == 128 && NST == 2) ? 4 : 1;
     static constexpr bool kSplit = (NPL >= 2);     // 2 or 3 planes: split tier
 };
 
@@ -845,7 +849,7 @@ __device__ __forceinline__ void gemm16_tile(const GemmProb& P, int bi, int i0, i
 }
 
 template <int NPL, int SRCA, int SRCB, int BM, int BN, int WM, int WN, int NST>
-__global__ void __launch_bounds__(64 * WM * WN)
+__global__ void __launch_bounds__(64 * WM * WN, (Gemm16Cfg<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>::kWavesPerEU))
 gemm16_kernel(GemmLaunch L)
 {
     using CFG = Gemm16Cfg<NPL, SRCA, SRCB, BM, BN, WM, WN, NST>;
