@@ -467,27 +467,75 @@ ozaki_gemm_wide_kernel(OzakiProb P, int tilesM, int tilesN, int totalTiles)
             else if (issued == 2) TB_OZ_WAIT_BARRIER(DMA_W);
             else TB_OZ_WAIT_BARRIER(0);
         }
-#pragma unroll 1
-        for (int b = 0; b < nblk; ++b) {
-            const int8_t* st = smem + (b % NST) * STAGE;
-            intx4_t af[S];
+        // D_{s+t} += A_s B_t (s, t < S; s + t < G): consecutive MFMAs update different D_g;
+        // int32 sums are exact, so their order does not matter
+        auto mfma_t = [&](const int8_t* st, int t, const intx4_t(&a)[S]) __attribute__((always_inline)) {
+            const intx4_t bt = *reinterpret_cast<const intx4_t*>(st + t * PLANE + offB);
 #pragma unroll
-            for (int s = 0; s < S; ++s) af[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offA);
-            // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
-            const bool ahead = b + NST - 1 < nblk;
-            if (ahead) issue(b + NST - 1);
-            // D_{s+t} += A_s B_t (s, t < S; s + t < G): consecutive MFMAs update different D_g;
-            // int32 sums are exact, so their order does not matter
+            for (int s = 0; s < S; ++s)
+                if (s + t < G)
+                    acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bt, a[s], acc[s + t], 0, 0, 0);
+        };
+        if constexpr (NST >= 3) {
+            // The block's barrier sits halfway through its MFMAs: after it (block b+1
+            // complete, block b-1 read by every wave) block b+NST-1's loads go into
+            // block b-1's slot and block b+1's A fragments are read while the second
+            // half's MFMAs run, so no wave waits for its A fragments after a barrier.
+            // first half: the slices t < TH of B, chosen so that the halves hold about
+            // the same number of MFMAs (slice t has min(S, G - t) of them)
+            constexpr int TH = [] {
+                int total = 0;
+                for (int t = 0; t < S; ++t) total += (G - t < S ? G - t : S);
+                int c = 0;
+                for (int t = 0; t < S; ++t) {
+                    c += (G - t < S ? G - t : S);
+                    if (2 * c >= total) return t + 1;
+                }
+                return S;
+            }();
+            constexpr int KEEPM = (NST - 3) * DMA_W;
+            intx4_t af[S], an[S];
+            if (nblk > 0) {
 #pragma unroll
-            for (int t = 0; t < S; ++t) {
-                const intx4_t bt = *reinterpret_cast<const intx4_t*>(st + t * PLANE + offB);
-#pragma unroll
-                for (int s = 0; s < S; ++s)
-                    if (s + t < G)
-                        acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bt, af[s], acc[s + t], 0, 0, 0);
+                for (int s = 0; s < S; ++s) af[s] = *reinterpret_cast<const intx4_t*>(smem + s * PLANE + offA);
             }
-            if (ahead) TB_OZ_WAIT_BARRIER(KEEP);
-            else TB_OZ_WAIT_BARRIER(0);
+#pragma unroll 1
+            for (int b = 0; b < nblk; ++b) {
+                const int8_t* st = smem + (b % NST) * STAGE;
+#pragma unroll
+                for (int t = 0; t < TH; ++t) mfma_t(st, t, af);
+                // (the MFMAs stay on their side of the barrier and of the reads after it)
+                __builtin_amdgcn_sched_barrier(0);
+                // block b+1's loads went out after the barrier of block b+2-NST; those of
+                // blocks b+3-NST .. b-1 may stay in flight when block b-1 issued its own
+                if (KEEPM > 0 && b + NST - 2 < nblk) TB_OZ_WAIT_BARRIER(KEEPM);
+                else TB_OZ_WAIT_BARRIER(0);
+                if (b + NST - 1 < nblk) issue(b + NST - 1);
+                // block b+1's A fragments (past the last block: block b's again, unused)
+                const int8_t* sn = smem + ((b + 1 < nblk ? b + 1 : b) % NST) * STAGE;
+#pragma unroll
+                for (int s = 0; s < S; ++s) an[s] = *reinterpret_cast<const intx4_t*>(sn + s * PLANE + offA);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int t = TH; t < S; ++t) mfma_t(st, t, af);
+#pragma unroll
+                for (int s = 0; s < S; ++s) af[s] = an[s];
+            }
+        } else {
+#pragma unroll 1
+            for (int b = 0; b < nblk; ++b) {
+                const int8_t* st = smem + (b % NST) * STAGE;
+                intx4_t af[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) af[s] = *reinterpret_cast<const intx4_t*>(st + s * PLANE + offA);
+                // slot (b-1) % NST was read by every wave before the barrier that ended block b-1
+                const bool ahead = b + NST - 1 < nblk;
+                if (ahead) issue(b + NST - 1);
+#pragma unroll
+                for (int t = 0; t < S; ++t) mfma_t(st, t, af);
+                if (ahead) TB_OZ_WAIT_BARRIER(KEEP);
+                else TB_OZ_WAIT_BARRIER(0);
+            }
         }
 
         // epilogue: the exactness chunk's D_g into FP64 in increasing g (on top of the
