@@ -7,8 +7,9 @@ also carries the plain FP16 tier, the split tier with FP64-residual refinement, 
 max-relative and normwise error of each tier against an FP64 host solve of a sample of
 rows, the dominant kernel's roofline and a host BLAS baseline on a bounded sample.
 
-One step = restore B from a resident copy (a device copy, 2 GiB of HBM traffic) +
-one tb_trsm call.  Inputs are resident in HBM before the timed region.
+One step = one tb_trsm call on a copy of B of its own (the solve overwrites B): the
+copies are made before the timed region, so inputs are resident in HBM when it starts
+(up to 32 copies; steps beyond that restore a copy inside the timed region).
 
 After the headline the same process measures the benchmark's other configurations and
 carries each, with its own errors, roofline and host baseline, under "configs"
@@ -192,18 +193,40 @@ def run_trsm(args, torch, tensorblas):
             h.trsm(tier, m, n, dL.data_ptr(), n, dB.data_ptr(), m, nb=nb)
         return f
 
+    # The solve overwrites B, so every timed step gets a copy of B of its own, made
+    # before the timed region (up to 32 of them, 1 GiB each at 16384^2; past that the
+    # steps reuse them and restore B inside the timed region).
+    stage = [torch.empty_like(dB0) for _ in range(min(max(args.steps, 3), 32))]
+
+    def timed_staged(tier, steps, warmup):
+        for _ in range(warmup):
+            step(tier)()
+        for b in stage[:steps]:
+            b.copy_(dB0)
+        h.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            b = stage[i % len(stage)]
+            if i >= len(stage):
+                b.copy_(dB0)
+            h.trsm(tier, m, n, dL.data_ptr(), n, b.data_ptr(), m, nb=nb)
+        h.synchronize()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
     log("bench: split tier (timed)")
-    ms_split = timed(h, step("split"), args.steps, args.warmup)
+    ms_split = timed_staged("split", args.steps, args.warmup)
     rows = np.random.default_rng(7).choice(m, 48, replace=False)
-    Xs = dB.cpu().numpy().T[rows]
+    Xs = stage[0].cpu().numpy().T[rows]
     log("bench: split tier (profiled step)")
     prof = profiled(h, step("split"))
     log("bench: fp32 FMA tier")
-    ms_fp32 = timed(h, step("fp32"), 2, 1)
-    Xf = dB.cpu().numpy().T[rows]
+    ms_fp32 = timed_staged("fp32", 2, 1)
+    Xf = stage[0].cpu().numpy().T[rows]
     log("bench: plain FP16 tier")
-    ms_plain = timed(h, step("plain"), 3, 1)
-    Xp = dB.cpu().numpy().T[rows]
+    ms_plain = timed_staged("plain", 3, 1)
+    Xp = stage[0].cpu().numpy().T[rows]
     log("bench: split tier + FP64-residual refinement")
 
     def refined():
