@@ -158,8 +158,8 @@ __device__ __forceinline__ void trtri_block(const T* __restrict__ Lq, long long 
 #pragma unroll
             for (int d = 0; d < kTrtriAhead; ++d) {
                 const int lr = lr0 + d;
+                const int r = u * 64 + lr;
                 if (lr < last) {
-                    const int r = u * 64 + lr;
                     const T rcp = rcpr[d];                                       // 1 / L(r, r)
                     T v[C];
 #pragma unroll
@@ -172,9 +172,13 @@ __device__ __forceinline__ void trtri_block(const T* __restrict__ Lq, long long 
                             for (int c = 0; c < C; ++c) acc[w][c] -= lq * v[c];
                         }
                     }
-                    fetch(r + kTrtriAhead, pre[d]);
-                    rcpr[d] = diag_rcp(r + kTrtriAhead, pre[d]);
                 }
+                // every lane issues every load (past the end: masked, clamped), and the
+                // reciprocal of a column's diagonal is formed half the prefetch distance
+                // after its loads, so no step waits for the loads it has just issued
+                fetch(r + kTrtriAhead, pre[d]);
+                const int dh = (d + kTrtriAhead / 2) % kTrtriAhead;
+                rcpr[dh] = diag_rcp(r + kTrtriAhead / 2, pre[dh]);
             }
         }
     }

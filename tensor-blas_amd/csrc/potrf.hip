@@ -91,22 +91,37 @@ potrf_leaf_kernel(T* __restrict__ A, long long lda, int s, long long off, int* _
                 lk0[q] = ca[k0 + q] * r0;
                 lk1[q] = (cb[k0 + q] - lk0[q] * l1) * r1;
             }
+            if (two && k0 > j + 1 && i0 > j + 1) {
+                // the whole tile is trailing: the rank-2 update alone, without the
+                // per-element selects (the same operations in the same order)
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii) {
-                const int i = i0 + ii;
+                for (int ii = 0; ii < 4; ++ii) {
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int k = k0 + kk;
-                    T v = a[ii][kk];
-                    if (k == j) {
-                        v = i == j ? d0 : (i > j ? li0[ii] : v);
-                    } else if (two && k == j + 1) {
-                        v = i == j + 1 ? d1 : (i > j + 1 ? li1[ii] : v);
-                    } else if (two && k > j + 1 && i > j + 1) {
+                    for (int kk = 0; kk < 4; ++kk) {
+                        T v = a[ii][kk];
                         v = v - li0[ii] * lk0[kk];
                         v = v - li1[ii] * lk1[kk];
+                        a[ii][kk] = v;
                     }
-                    a[ii][kk] = v;
+                }
+            } else {
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) {
+                    const int i = i0 + ii;
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const int k = k0 + kk;
+                        T v = a[ii][kk];
+                        if (k == j) {
+                            v = i == j ? d0 : (i > j ? li0[ii] : v);
+                        } else if (two && k == j + 1) {
+                            v = i == j + 1 ? d1 : (i > j + 1 ? li1[ii] : v);
+                        } else if (two && k > j + 1 && i > j + 1) {
+                            v = v - li0[ii] * lk0[kk];
+                            v = v - li1[ii] * lk1[kk];
+                        }
+                        a[ii][kk] = v;
+                    }
                 }
             }
             // the tiles holding columns j+2, j+3 publish them (static indices only: a
