@@ -45,20 +45,34 @@ ozaki_slice_kernel(const double* __restrict__ X, long long ld, int mode, int row
     const double* Xq = X + (long long)q * sx;
     const int r0 = blockIdx.y * kSliceTile, k0 = blockIdx.x * kSliceTile;
     const int tid = threadIdx.x;
-    // load: mode N / TriL / SymL-lower read rows contiguous, mode T reads k contiguous
-    for (int e = tid; e < kSliceTile * kSliceTile; e += 256) {
-        int rl, kl;
-        if (mode == kOpT) { kl = e % kSliceTile; rl = e / kSliceTile; }
-        else { rl = e % kSliceTile; kl = e / kSliceTile; }
-        const int r = r0 + rl, k = k0 + kl;
-        double v = 0.0;
-        if (r < rows && k < K) {
-            if (mode == kOpN) v = Xq[(long long)r + (long long)k * ld];
-            else if (mode == kOpT) v = Xq[(long long)k + (long long)r * ld];
-            else if (r >= k) v = Xq[(long long)r + (long long)k * ld];
-            else if (mode == kOpSymL) v = Xq[(long long)k + (long long)r * ld];
+    // load: mode N / TriL / SymL-lower read rows contiguous, mode T reads k contiguous.
+    // Every thread issues its 16 loads (unconditionally, at clamped addresses, masked
+    // afterwards) before the first LDS store, so they are in flight together.  With
+    // K = 0 the operand may be unreferenced and no load is issued.
+    constexpr int kPer = kSliceTile * kSliceTile / 256;
+    static_assert(kSliceTile == 64, "element e = tid + 256 it sits at (e % 64, e / 64)");
+    const bool tr = mode == kOpT;
+    const int ea = tid & 63, eb = tid >> 6;
+    double v[kPer];
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) v[it] = 0.0;
+    if (K > 0) {
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int rl = tr ? eb + 4 * it : ea, kl = tr ? ea : eb + 4 * it;
+            const int r = r0 + rl, k = k0 + kl;
+            // X(r, k) stored (N, or the lower triangle of TriL / SymL), else X(k, r)
+            const bool direct = mode == kOpN || (!tr && r >= k);
+            const bool in = r < rows && k < K && (direct || tr || mode == kOpSymL);
+            const long long off = direct ? (long long)r + (long long)k * ld
+                                         : (long long)k + (long long)r * ld;
+            v[it] = masked_load(Xq + (in ? off : 0ll), in);
         }
-        t[rl][kl] = v;
+    }
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+        const int rl = tr ? eb + 4 * it : ea, kl = tr ? ea : eb + 4 * it;
+        t[rl][kl] = v[it];
     }
     __syncthreads();
     // each thread: one row, 16 consecutive k; rows past `rows` (up to the next multiple
