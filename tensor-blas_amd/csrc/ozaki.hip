@@ -357,7 +357,12 @@ constexpr int ozw_stages()
     return 4 * (6 * S * 1024) <= 160 * 1024 ? 4 : (3 * (6 * S * 1024) <= 160 * 1024 ? 3 : 2);
 }
 
-// Wide tile g (128 x 64) of the lower part of C, walked in bands of kBand tile rows:
+// tile rows per band of the wide kernel's tile order: the 32 workgroups of an XCD then
+// cover 4 x 8 tiles (512 rows of A, 512 columns of B per k block); bands of 8 and 16
+// rows measured 1-2 % and 4-7 % slower on SYR2K / SYMM 32768 x 8192
+constexpr int kBandW = 4;
+
+// Wide tile g (128 x 64) of the lower part of C, walked in bands of kBandW tile rows:
 // inside a band column by column (rows fastest), first the columns left of the band's
 // diagonal (every row of the band), then those it crosses, column tj needing rows
 // ti >= tj / 2 (128 ti + 127 >= 64 tj).
@@ -365,7 +370,7 @@ __device__ __forceinline__ void lower_wide_banded(int g, int T, int tn, int& ti,
 {
     int r0 = 0;
     for (;;) {
-        const int gb = min(kBand, T - r0);
+        const int gb = min(kBandW, T - r0);
         const int left = min(2 * r0, tn);
         int cnt = left * gb;
         for (int c = left; c < min(2 * (r0 + gb), tn); ++c) cnt += r0 + gb - c / 2;
@@ -373,7 +378,7 @@ __device__ __forceinline__ void lower_wide_banded(int g, int T, int tn, int& ti,
         g -= cnt;
         r0 += gb;
     }
-    const int gb = min(kBand, T - r0);
+    const int gb = min(kBandW, T - r0);
     const int left = min(2 * r0, tn);
     if (g < left * gb) {
         tj = g / gb;
@@ -416,9 +421,9 @@ ozaki_gemm_wide_kernel(OzakiProb P, int tilesM, int tilesN, int totalTiles)
         if (P.lowerC) {
             lower_wide_banded(g, tilesM, tilesN, ti, tj);
         } else {
-            const int per = kBand * tilesN;
-            const int first = (g / per) * kBand;
-            const int gsz = min(tilesM - first, kBand);
+            const int per = kBandW * tilesN;
+            const int first = (g / per) * kBandW;
+            const int gsz = min(tilesM - first, kBandW);
             const int in = g - (g / per) * per;
             ti = first + in % gsz;
             tj = in / gsz;
