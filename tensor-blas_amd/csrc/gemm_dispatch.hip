@@ -114,7 +114,9 @@ int capped_grid(const void* kernel, int threads, int total, int max_cus)
 
 TileShape gemm_tile_shape(int tier, int shape)
 {
-    if (shape == kShapeLeaf) return TileShape{64, 256};
+    // FP64 leaves: 32-row tiles, so that a solve of 8192 rows fills 256 workgroups
+    // rather than 128
+    if (shape == kShapeLeaf) return TileShape{tier == kMfmaF64 ? 32 : 64, 256};
     if (shape == kShapeWide) return TileShape{256, 256};
     return TileShape{128, 128};
 }
@@ -147,7 +149,7 @@ hipError_t gemm_launch(int tier, int npl, int shape, int srcA, int srcB, const G
                                dim3(g64::NT), 0, s, L);
             return hipGetLastError();
         }
-        return shape == kShapeLeaf ? run<kMfmaF64, 64, 256, 16, 1, 4>(L, s, max_cus)
+        return shape == kShapeLeaf ? run<kMfmaF64, 32, 256, 16, 1, 4>(L, s, max_cus)
                                    : run<kMfmaF64, 128, 128, 16, 2, 2>(L, s, max_cus);
     default:
         return hipErrorInvalidValue;

@@ -13,8 +13,9 @@ namespace tb {
 // ------------------------------------------------------------- grouped GEMM
 enum GemmShape : int {
     kShapeSquare = 0,  // 128 x 128 tiles: off-diagonal updates and whole routines
-    kShapeLeaf = 1,    // 64 x 256 tiles: a workgroup owns all n <= 256 columns of its rows,
-                       // so C may alias op(A) (in-place TRSM leaf X := B * inv(L)^T)
+    kShapeLeaf = 1,    // 64 x 256 tiles (FP64: 32 x 256): a workgroup owns all n <= 256
+                       // columns of its rows, so C may alias op(A) (in-place TRSM leaf
+                       // X := B * inv(L)^T)
     kShapeWide = 2,    // 256 x 256 tiles (plain and 2-plane split): full or lower C,
                        // both operands in FP16 planes; chosen by GemmQueue::flush for
                        // launches with enough tiles to fill the device

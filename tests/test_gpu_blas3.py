@@ -249,7 +249,7 @@ TRSM_TOL = {"plain": 2.0 ** -9, "split": 1e-6, "fp32": 1e-6, "fp64": 1e-13, "oza
 
 @pytest.mark.parametrize("tier", ["plain", "split", "fp32", "fp64", "ozaki"])
 @pytest.mark.parametrize("m,n,nb", [(1, 1, 256), (7, 300, 256), (64, 2048, 256), (130, 2048, 512),
-                                    (33, 4100, 256)])
+                                    (33, 4100, 256), (8200, 520, 256)])
 def test_trsm(h, tier, m, n, nb):
     g = rng(m * 5 + n + nb)
     dt = tier_dtype(tier)
