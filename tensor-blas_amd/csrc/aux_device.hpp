@@ -35,15 +35,17 @@ __device__ __forceinline__ int expo(double x)
 constexpr int kRowChunk = 256;    // k per block of the row-major pass
 constexpr int kColChunk = 1024;   // k per block of the k-contiguous pass
 
-// rows contiguous across threads: one thread per row of rows rb*256.., k chunk kb
+// rows contiguous across threads: one thread per row of rows rb*256.., k chunk kb of
+// `chunk` columns
 template <typename T>
 __device__ __forceinline__ void rowexp_mn_block(const T* __restrict__ X, long long ld, int rows,
-                                                int K, bool lower_only, int* out, int rb, int kb)
+                                                int K, bool lower_only, int* out, int rb, int kb,
+                                                int chunk = kRowChunk)
 {
     const int r = rb * 256 + threadIdx.x;
     if (r >= rows) return;
-    const int k0 = kb * kRowChunk;
-    int k1 = min(K, k0 + kRowChunk);
+    const int k0 = kb * chunk;
+    int k1 = min(K, k0 + chunk);
     if (lower_only) k1 = min(k1, r + 1);
     int e = kExpZero;
     int k = k0;

@@ -16,10 +16,10 @@ namespace {
 template <typename T>
 __global__ void __launch_bounds__(256)
 rowexp_mn_kernel(const T* __restrict__ X, long long ld, int rows, int K, bool lower_only,
-                 long long sx, int* out, long long so)
+                 long long sx, int* out, long long so, int chunk)
 {
     rowexp_mn_block(X + (long long)blockIdx.z * sx, ld, rows, K, lower_only,
-                    out + (long long)blockIdx.z * so, blockIdx.x, blockIdx.y);
+                    out + (long long)blockIdx.z * so, blockIdx.x, blockIdx.y, chunk);
 }
 
 template <typename T>
@@ -208,14 +208,20 @@ hipError_t rowexp(bool f64, const void* X, long long ld, int mode, int rows, int
 {
     if (rows <= 0 || K <= 0 || nb <= 0) return hipSuccess;
     if (mode == kOpN || mode == kOpTriL || mode == kOpSymL) {
-        dim3 g((rows + 255) / 256, (K + kRowChunk - 1) / kRowChunk, nb);
+        // k chunks of 256 columns, halved (down to 32) while the grid has fewer than 1024
+        // workgroups: a thread's chunk is a chain of dependent rounds of 8 loads, so
+        // small passes (the 256-column blocks of a TRSM recursion) need the shorter chains
+        const long long rb = (rows + 255) / 256;
+        int chunk = kRowChunk;
+        while (chunk > 32 && rb * ((K + chunk - 1) / chunk) * nb < 1024) chunk /= 2;
+        dim3 g((unsigned)rb, (K + chunk - 1) / chunk, nb);
         const bool lower = mode != kOpN;
         if (f64)
             hipLaunchKernelGGL(rowexp_mn_kernel<double>, g, dim3(256), 0, s,
-                               (const double*)X, ld, rows, K, lower, sx, out, so);
+                               (const double*)X, ld, rows, K, lower, sx, out, so, chunk);
         else
             hipLaunchKernelGGL(rowexp_mn_kernel<float>, g, dim3(256), 0, s,
-                               (const float*)X, ld, rows, K, lower, sx, out, so);
+                               (const float*)X, ld, rows, K, lower, sx, out, so, chunk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess || mode != kOpSymL) return e;
     }
