@@ -55,8 +55,9 @@ int ozaki_gemm_host(tb_handle_s* h, long m, long n, long k, double alpha, const 
     int8_t* sB = reinterpret_cast<int8_t*>(ws + oSB);
     // op(B) columns are the rows of op(B)^T
     const int rmB = flip_mode(modeB);
-    TB_HIP(fill_i32(eA, m, kExpZero, st));
-    TB_HIP(fill_i32(eB, n, kExpZero, st));
+    // eB follows eA in the workspace (reserved in that order): one fill covers both and
+    // the alignment gap between them
+    TB_HIP(fill_i32(eA, (long)(eB - eA) + n, kExpZero, st));
     if (k > 0) {
         TB_HIP(rowexp(true, A, lda, modeA, (int)m, (int)k, 1, 0, eA, 0, st));
         TB_HIP(rowexp(true, B, ldb, rmB, (int)n, (int)k, 1, 0, eB, 0, st));
