@@ -88,8 +88,9 @@ double req_flops(const tb_request& r)
 // Time alone on MI355X, t = a + b n + flops / c, per routine and tier (plain, split,
 // Ozaki): a covers launches and conversions, b the dependent chain of a recursion
 // (TRSM's n/nb leaves), c the rate of its products.  Fitted (non-negative least
-// squares on relative error) to isolated calls of every routine and tier at n = 128 ..
-// 8192 (k = n, m = n); median error 2-7 %, TRSM 11-18 %.
+// squares on relative error) to the isolated times of a stream of 500 requests, every
+// routine and tier, n log-uniform in 128 .. 8192 (k = n, m = n); median error 2-7 %,
+// TRSM 11-16 %.
 // The FP32 and FP64 comparator tiers use a fixed cost and a flat rate.
 struct TimeModel {
     double a_us, b_us_per_n, c_tflops;
@@ -97,11 +98,11 @@ struct TimeModel {
 double est_us(const tb_request& r)
 {
     static const TimeModel model[5][3] = {
-        /* SYRK  */ {{26.0, 0.0055, 651.0}, {48.4, 0.0163, 304.0}, {60.2, 0.0305, 47.2}},
-        /* SYR2K */ {{29.9, 0.0095, 680.0}, {82.2, 0.0234, 279.0}, {88.2, 0.115, 50.4}},
-        /* TRMM  */ {{27.6, 0.0096, 409.0}, {61.0, 0.0115, 153.0}, {65.4, 0.0087, 24.7}},
-        /* TRSM  */ {{102.9, 0.381, 1e9}, {99.0, 0.452, 453.0}, {1.2, 0.973, 92.7}},
-        /* SYMM  */ {{28.1, 0.0090, 691.0}, {60.2, 0.0202, 297.0}, {65.4, 0.0163, 46.9}},
+        /* SYRK  */ {{27.8, 0.0062, 716.0}, {39.5, 0.0141, 296.5}, {45.8, 0.0279, 72.2}},
+        /* SYR2K */ {{30.2, 0.0130, 909.3}, {51.8, 0.0243, 276.3}, {83.2, 0.0422, 71.3}},
+        /* TRMM  */ {{29.5, 0.0099, 406.5}, {45.9, 0.0181, 172.8}, {48.4, 0.0267, 41.6}},
+        /* TRSM  */ {{70.4, 0.3406, 1e9}, {61.0, 0.3890, 390.2}, {8.1, 0.7155, 158.0}},
+        /* SYMM  */ {{32.3, 0.0086, 702.5}, {50.5, 0.0153, 299.8}, {55.7, 0.0152, 78.8}},
     };
     const double f = req_flops(r);
     if (r.tier == TB_TIER_FP32) return 50.0 + f / (70.0 * 1e6);
