@@ -1052,7 +1052,7 @@ def main():
                     help="serve: estimated time alone below which a request is small")
     ap.add_argument("--small-slots", type=int, default=0,
                     help="serve: slots kept for small requests")
-    ap.add_argument("--order", default="shortest", choices=["fifo", "shortest", "shortest-small"])
+    ap.add_argument("--order", default="fifo", choices=["fifo", "shortest", "shortest-small"])
     ap.add_argument("--reserve-cus", type=int, default=0)
     ap.add_argument("--partition", type=int, default=0, choices=[0, 1, 2],
                     help="serve: --reserve-cus CUs left free of large requests, or with "
