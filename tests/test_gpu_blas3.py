@@ -335,6 +335,29 @@ def test_ozaki_gemm_bit_exact(h, S, extra, m, n, k, phi):
     assert np.array_equal(C, R), np.max(np.abs(C - R))
 
 
+@pytest.mark.parametrize("op", ["gemm", "syrk"])
+def test_ozaki_k_zero_scales_c(h, op):
+    """k = 0: C := beta C exactly (the lower triangle for SYRK); the operand buffers hold
+    only NaN, so any value read from them would show in C."""
+    g = rng(4242)
+    n = 70
+    C0 = spread(g, (n, n), 0.5)
+    dA, lda = to_dev(np.zeros((n, 0)), n)
+    dC, ldc = to_dev(C0, n + 1)
+    if op == "gemm":
+        dB, ldb = to_dev(np.zeros((0, n)), 1)
+        h.gemm("ozaki", 0, 0, n, n, 0, 1.25, dA.data_ptr(), lda, dB.data_ptr(), ldb, -0.5,
+               dC.data_ptr(), ldc)
+    else:
+        h.syrk("ozaki", n, 0, 1.25, dA.data_ptr(), lda, -0.5, dC.data_ptr(), ldc)
+    h.synchronize()
+    C = from_dev(dC, n, n, ldc)
+    R = -0.5 * C0
+    if op == "syrk":
+        R = np.tril(R) + np.triu(C0, 1)
+    assert np.array_equal(C, R)
+
+
 @pytest.mark.parametrize("S,extra", [(9, 0), (8, 1)])
 def test_ozaki_long_k_bit_exact(h, S, extra):
     """K past one exactness chunk with beta = 0 runs as one launch per chunk (each chunk's
